@@ -1,0 +1,200 @@
+"""Benchmark of the element-assembly hot path on one MI355X.
+
+A step is one residual + CSR Jacobian assembly (the residual and stiffness calls of one Newton iteration,
+src/Solvers.jl:133-141) over the whole mesh, inputs resident in HBM: DOF update, element kernel (FP32), FP64
+reductions into the residual field and the CSR values. Default workload: 3D scalar Poisson on the unit cube,
+128^3 hex8 (2,097,152 elements), f = 1, u ~ U[-1, 1] per node.
+
+Prints one JSON line: elements/s, ms/step, the dominant kernel's roofline fraction (HIP-event timing on the
+assembler's stream), relative errors against the FP64 oracle on a bounded sample mesh, and the oracle's CPU
+throughput on that sample (cpu_baseline, kind "port": the C++ FP64 restatement of the reference's CPU algorithm).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__ as ge  # noqa: E402
+
+METRIC = "residual+Jacobian elements/sec: FP32/learned vs FP64 plain at stated rel err"
+HBM_PEAK_GBS = 8000.0
+
+# per-config full sizes and the bounded sample used for parity and the CPU baseline
+SIZES = {"poisson_hex8": (128, 56), "neohookean_hex8": (96, 32), "j2_hex8": (64, 28), "neohookean_tet10": (80, 14),
+         "poisson_quad4": (64, 64)}
+# SURVEY algorithmic bytes per element for residual + tangent (int32 conn, FP32 coords, FP64 u, FP64 outputs written
+# once, int32 COO->CSR map of NDOF^2 per element, FP32 state)
+SURVEY_BYTES = {"poisson_hex8": 534.4, "neohookean_hex8": 4362.2, "j2_hex8": 4821.4, "neohookean_tet10": 6502.3}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_assembler(fec, wl, device=0):
+    V = fec.FunctionSpace(wl.mesh, "H1Field", "Lagrange")
+    var = fec.ScalarFunction(V, "u") if wl.nf == 1 else fec.VectorFunction(V, "displ")
+    asm = fec.SparseMatrixAssembler(var, sparse_matrix_type="csr", device=device)
+    bcs = []
+    if len(wl.dirichlet_dofs):
+        nodes = np.unique((wl.dirichlet_dofs - 1) // wl.nf + 1)
+        wl.mesh.nodeset_nodes["_bc"] = nodes
+        bcs = [fec.DirichletBC(n, lambda X, t: 0.0, nset_name="_bc") for n in var.names]
+    p = fec.create_parameters(wl.mesh, asm, wl.physics, dirichlet_bcs=bcs)
+    return asm, p
+
+
+def kernel_bytes(wl, asm):
+    """Algorithmic HBM bytes per launch of each kernel of the two-phase assembly (compulsory traffic)."""
+    ne, nnpe = wl.mesh.conn.shape
+    nn, nf = wl.mesh.num_nodes, wl.nf
+    nd = wl.mesh.coords.shape[1]
+    ndof = nnpe * nf
+    npairs = nnpe * (nnpe + 1) // 2
+    nnzb = asm.nnz // (nf * nf)
+    nfree = len(asm.dof.unknown_dofs) // nf
+    ncontrib = ne * nnpe * nnpe  # no Dirichlet DOFs in the benchmark workloads
+    state = 2 * 4 * 7 * 8 * ne if wl.physics.kind == 3 else 0
+    return {
+        "element": 4 * nnpe * ne + 2 * 4 * nd * nn + 8 * nf * nn + 4 * ndof * ne + 4 * npairs * nf * nf * ne + state,
+        "reduce_matrix": 8 * (nfree + 1) + 8 * (nnzb + 1) + 4 * ncontrib + 4 * npairs * nf * nf * ne + 8 * asm.nnz,
+        "reduce_vector": 4 * (nn + 1) + 4 * ne * nnpe + 4 * ndof * ne + 8 * nf * nn,
+    }
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def oracle_step(orc, wl, d2u, nu):
+    m = wl.mesh
+    kind = {"QUAD4": orc.QUAD4, "HEX8": orc.HEX8, "TET10": orc.TET10}[m.element_type]
+    props = wl.physics.props()
+    st = np.zeros(m.num_elements * 8 * 7) if wl.physics.kind == orc.J2 else None
+    R = orc.assemble_residual(kind, m.coords, m.conn, wl.nf, wl.physics.kind, props, wl.U, st,
+                              None if st is None else np.zeros_like(st))
+    rp, cv, vals = orc.stiffness_csr(kind, m.coords, m.conn, wl.nf, wl.physics.kind, props, wl.U, d2u, nu, st)
+    return R, vals
+
+
+def sample_checks(fec, orc, name, n_sample, threads):
+    """GPU vs oracle relative errors and the oracle's CPU time on the same sample mesh."""
+    import torch
+    wl = fec.workloads.CONFIGS[name](n_sample)
+    asm, p = build_assembler(fec, wl)
+    Uu = torch.from_numpy(wl.U).to(asm.device)
+    fec.assemble_vector_and_stiffness_(asm, Uu, p)
+    R_gpu = asm.residual_storage.cpu().numpy()
+    K_gpu = asm.stiffness_storage.cpu().numpy()
+    N = wl.nf * wl.mesh.num_nodes
+    unknown, d2u = orc.update_dofs(N, np.zeros(0, np.int64))
+    orc.set_threads(threads)
+    Is, Js, slots, perm = orc.pattern(wl.nf, wl.mesh.conn, d2u)  # setup (update_dofs!), not timed
+    oracle_step(orc, wl, d2u, len(unknown))  # warm-up
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        R, vals = oracle_step(orc, wl, d2u, len(unknown))
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"R": rel(R_gpu, R), "K": rel(K_gpu, vals)}, wl.mesh.num_elements, t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="poisson_hex8", choices=sorted(SIZES))
+    ap.add_argument("--n", type=int, default=None, help="elements per direction (default: the config's size)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the sample parity/CPU legs (profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU assembly is a design in "
+              "DESIGN.md, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    fec = ge.load_package()
+    name = args.workload
+    n_full, n_sample = SIZES[name]
+    n = args.n or n_full
+    log(f"[bench] building {name} n={n}")
+    t0 = time.time()
+    wl = fec.workloads.CONFIGS[name](n)
+    asm, p = build_assembler(fec, wl)
+    Uu = torch.from_numpy(wl.U[asm.dof.unknown_dofs - 1]).to(asm.device)
+    log(f"[bench] setup {time.time() - t0:.1f}s  NE={wl.num_elements} nnz={asm.nnz}")
+
+    for _ in range(args.warmup):
+        fec.assemble_vector_and_stiffness_(asm, Uu, p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fec.assemble_vector_and_stiffness_(asm, Uu, p)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    log(f"[bench] {dt * 1e3:.3f} ms/step")
+
+    # per-kernel durations with HIP events on the assembler's stream
+    fec.reset_profile(asm)
+    fec.set_profiling(asm, True)
+    for _ in range(max(3, min(args.steps, 10))):
+        fec.assemble_vector_and_stiffness_(asm, Uu, p)
+    fec.set_profiling(asm, False)
+    prof = fec.profile(asm)
+    kb = kernel_bytes(wl, asm)
+    kernels = {k: {"avg_ms": v[1] / v[0], "bytes": kb.get(k)} for k, v in prof.items()}
+    for k, v in kernels.items():
+        if v["bytes"]:
+            v["gbs"] = v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    ach = kernels[dom]["gbs"]
+    log("[bench] kernels: " + ", ".join(f"{k} {v['avg_ms']:.3f} ms {v.get('gbs', 0):.0f} GB/s" for k, v in kernels.items()))
+
+    out = {
+        "metric": METRIC,
+        "value": wl.num_elements / dt,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{name} {n}^3" if wl.mesh.num_dims == 3 else f"{name} {n}^2",
+                   "elements": int(wl.num_elements), "nnz": int(asm.nnz), "step": "residual + CSR Jacobian",
+                   "element_math": "fp32", "reductions": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None},
+        "kernels": kernels,
+    }
+    if name in SURVEY_BYTES:
+        eff = SURVEY_BYTES[name] * wl.num_elements / dt / 1e9
+        out["step_roofline"] = {"bytes_per_element": SURVEY_BYTES[name], "achieved": eff, "frac": eff / HBM_PEAK_GBS}
+
+    if not args.no_check:
+        orc = ge.load_oracle()
+        threads = min(16, os.cpu_count() or 1)
+        errs, ne_s, t_cpu = sample_checks(fec, orc, name, n_sample, threads)
+        out["rel_err"] = {"residual": errs["R"], "jacobian": errs["K"],
+                          "sample": f"{name} n={n_sample} ({ne_s} elements) vs FP64 oracle"}
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = {"value": ne_s / t_cpu, "unit": "elements/s", "cores": threads, "kind": "port",
+                                   "sample": f"{name} n={n_sample}: {ne_s} elements, residual + element stiffness + "
+                                             f"sparse! COO->CSR, median of 3 after 1 warm-up"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

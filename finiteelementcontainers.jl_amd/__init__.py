@@ -1,0 +1,17 @@
+"""MI355X element-assembly path of FiniteElementContainers.jl: residual, CSR Jacobian and Jacobian actions
+assembled by hand-written gfx950 HIP kernels in libfecx.so (C ABI: include/fecx.h).
+
+The package directory name contains a dot, so load it by path (``__graft_entry__.load_package()``).
+"""
+from ._lib import FECXError, exported_symbols, LIB_PATH  # noqa: F401
+from .mesh import StructuredMesh, TetMesh, kuhn_tet10_mesh, boundary_nodes  # noqa: F401
+from .assembler import (  # noqa: F401
+    Poisson, Mechanics, NeoHookean, J2Plasticity, ThreeDimensional,
+    residual, stiffness, stiffness_action,
+    FunctionSpace, ScalarFunction, VectorFunction, DofManager, DirichletBC, DirichletBCs,
+    CSRMatrix, SparseMatrixAssembler, HostPattern, Parameters, create_parameters, create_unknowns, create_field,
+    assemble_vector_, assemble_stiffness_, assemble_matrix_, assemble_vector_and_stiffness_,
+    assemble_matrix_action_, assemble_matrix_free_action_, assemble_matrix_free_action_full_, hvp,
+    set_profiling, profile, reset_profile,
+)
+from . import rng, workloads  # noqa: F401

@@ -1,0 +1,513 @@
+// C-ABI entry points of libfecx.so (declared in include/fecx.h, each with its reference citation).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "fecx_internal.h"
+
+namespace {
+
+struct ApiError : std::runtime_error {
+  int code;
+  ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                      \
+  do {                                                                                                 \
+    hipError_t e_ = (x);                                                                               \
+    if (e_ != hipSuccess) throw ApiError(FECX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class F>
+int guard(fecx_ctx* ctx, F f) {
+  if (!ctx) return FECX_ERR_ARG;
+  ctx->err.clear();
+  try {
+    if (ctx->device >= 0) HIPCHK(hipSetDevice(ctx->device));
+    f();
+    return FECX_OK;
+  } catch (const ApiError& e) {
+    ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return FECX_ERR_ARG;
+  }
+}
+
+void require(bool c, int code, const std::string& msg) {
+  if (!c) throw ApiError(code, msg);
+}
+
+// device buffers; a host-only context (device -1) allocates nothing
+void* dev(fecx_ctx* ctx, const std::string& name, size_t bytes) {
+  if (ctx->device < 0) return nullptr;
+  DevBuf& b = ctx->buf[name];
+  if (b.bytes != bytes) {
+    if (b.p) HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+    if (bytes) HIPCHK(hipMalloc(&b.p, bytes));
+    b.bytes = bytes;
+  }
+  return b.p;
+}
+
+template <class T>
+T* upload(fecx_ctx* ctx, const std::string& name, const std::vector<T>& v) {
+  T* p = (T*)dev(ctx, name, v.size() * sizeof(T));
+  if (p && !v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+template <class T>
+T* get(fecx_ctx* ctx, const std::string& name) {
+  auto it = ctx->buf.find(name);
+  return it == ctx->buf.end() ? nullptr : (T*)it->second.p;
+}
+
+// kernel profiling with HIP events on the context's stream
+struct Prof {
+  fecx_ctx* ctx;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  explicit Prof(fecx_ctx* c) : ctx(c) {}
+  template <class F>
+  void run(const char* name, F f) {
+    if (!ctx->profiling) { f(); return; }
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, ctx->stream));
+    f();
+    HIPCHK(hipEventRecord(b, ctx->stream));
+    ev.push_back({name, {a, b}});
+  }
+  void finish() {
+    for (auto& x : ev) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, x.second.first, x.second.second));
+      auto& pe = ctx->prof[x.first];
+      if (pe.launches == 0 && std::find(ctx->prof_names.begin(), ctx->prof_names.end(), x.first) == ctx->prof_names.end())
+        ctx->prof_names.push_back(x.first);
+      pe.launches++;
+      pe.ms += ms;
+      (void)hipEventDestroy(x.second.first);
+      (void)hipEventDestroy(x.second.second);
+    }
+    ev.clear();
+  }
+};
+
+__global__ void zero_int_kernel(int* p) { *p = 0; }
+
+void check_device(fecx_ctx* ctx) {
+  require(ctx->device >= 0, FECX_ERR_STATE, "host-only context (device -1) cannot run kernels");
+}
+
+void check_ready(fecx_ctx* ctx) {
+  check_device(ctx);
+  require(ctx->has_mesh, FECX_ERR_STATE, "fecx_set_mesh has not been called");
+  require(ctx->has_dofs, FECX_ERR_STATE, "fecx_set_dofs has not been called");
+  require(ctx->has_pattern, FECX_ERR_STATE, "fecx_build_pattern has not been called");
+  require(ctx->has_material, FECX_ERR_STATE, "fecx_set_material has not been called");
+}
+
+fecx::ElementArgs element_args(fecx_ctx* ctx, const double* U, const double* V) {
+  fecx::ElementArgs a{};
+  a.ne = ctx->ne;
+  a.conn = get<int32_t>(ctx, "conn");
+  a.xhi = get<float>(ctx, "xhi");
+  a.xlo = get<float>(ctx, "xlo");
+  a.U = U;
+  a.V = V;
+  a.st_old = get<float>(ctx, "state_old");
+  a.st_new = get<float>(ctx, "state_new");
+  a.Rel = get<float>(ctx, "Rel");
+  a.Kel = get<float>(ctx, "Kel");
+  a.flag = get<int>(ctx, "flag");
+  a.mp = ctx->mp;
+  return a;
+}
+
+fecx::ReduceArgs reduce_args(fecx_ctx* ctx) {
+  fecx::ReduceArgs r{};
+  r.nfree = ctx->hp.nfree;
+  r.nn = ctx->nn;
+  r.bptr = get<int64_t>(ctx, "bptr");
+  r.boff = get<int64_t>(ctx, "boff");
+  r.bent = get<uint32_t>(ctx, "bent");
+  r.voff = get<int32_t>(ctx, "voff");
+  r.vent = get<int32_t>(ctx, "vent");
+  r.Kel = get<float>(ctx, "Kel");
+  r.Rel = get<float>(ctx, "Rel");
+  return r;
+}
+
+// runs the element kernel for `mode`, then the requested reductions; syncs and checks the flag
+void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double* R, double* vals) {
+  check_ready(ctx);
+  require(U != nullptr, FECX_ERR_ARG, "U is NULL");
+  Prof prof(ctx);
+  int* flag = get<int>(ctx, "flag");
+  zero_int_kernel<<<1, 1, 0, ctx->stream>>>(flag);
+  fecx::ElementArgs ea = element_args(ctx, U, V);
+  if (ctx->mp.phys == FECX_J2) require(ea.st_old != nullptr, FECX_ERR_STATE, "state not allocated");
+  int rc = 0;
+  prof.run("element", [&] { rc = fecx::launch_element(ctx->kind, ctx->nf, mode, ea, ctx->stream); });
+  require(rc == 0, FECX_ERR_UNSUPPORTED, "no element kernel for this element kind / nf / physics");
+  HIPCHK(hipGetLastError());
+  fecx::ReduceArgs ra = reduce_args(ctx);
+  if (mode & (fecx::MODE_R | fecx::MODE_ACTION_MF | fecx::MODE_ACTION_ASM)) {
+    ra.R = R;
+    prof.run("reduce_vector", [&] { rc = fecx::launch_reduce_vector(ctx->nf, ra, ctx->stream); });
+    HIPCHK(hipGetLastError());
+  }
+  if (mode & fecx::MODE_K) {
+    ra.vals = vals;
+    prof.run("reduce_matrix", [&] { rc = fecx::launch_reduce_matrix(ctx->nf, ra, ctx->stream); });
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  prof.finish();
+  int hflag = 0;
+  HIPCHK(hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost));
+  require(hflag == 0, FECX_ERR_NUMERIC, "element kernel met a non-positive Jacobian determinant or a NaN");
+}
+
+}  // namespace
+
+extern "C" {
+
+int fecx_create(fecx_ctx** out, int device) {
+  if (!out) return FECX_ERR_ARG;
+  *out = nullptr;
+  if (device < -1) return FECX_ERR_ARG;
+  if (device >= 0) {  // device -1: host-only context (pattern and index maps only)
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device >= n) return FECX_ERR_HIP;
+  }
+  fecx_ctx* ctx = new fecx_ctx();
+  ctx->device = device;
+  int rc = guard(ctx, [&] { (void)dev(ctx, "flag", sizeof(int)); });
+  if (rc) { delete ctx; return rc; }
+  *out = ctx;
+  return FECX_OK;
+}
+
+int fecx_destroy(fecx_ctx* ctx) {
+  if (!ctx) return FECX_OK;
+  if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+  for (auto& kv : ctx->buf)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  delete ctx;
+  return FECX_OK;
+}
+
+const char* fecx_last_error(const fecx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fecx_set_stream(fecx_ctx* ctx, void* s) {
+  return guard(ctx, [&] { ctx->stream = (hipStream_t)s; });
+}
+
+int fecx_set_mesh(fecx_ctx* ctx, int elem_kind, int nd, int64_t nn, const double* coords, int64_t ne,
+                  const int64_t* conn) {
+  return guard(ctx, [&] {
+    fecx::ElemInfo ei = fecx::elem_info(elem_kind);
+    require(ei.kind >= 0, FECX_ERR_UNSUPPORTED, "unsupported element kind");
+    require(nd == ei.nd, FECX_ERR_ARG, "coordinate dimension does not match the element kind");
+    require(nn > 0 && ne > 0 && coords && conn, FECX_ERR_ARG, "empty mesh");
+    require(nn < (int64_t(1) << 31) && ne * ei.nnpe < (int64_t(1) << 31), FECX_ERR_ARG, "mesh exceeds int32 indexing");
+    ctx->kind = elem_kind; ctx->nd = nd; ctx->nnpe = ei.nnpe; ctx->nq = ei.nq; ctx->nn = nn; ctx->ne = ne;
+    ctx->conn0.resize(ne * ei.nnpe);
+    for (int64_t i = 0; i < ne * ei.nnpe; ++i) {
+      require(conn[i] >= 1 && conn[i] <= nn, FECX_ERR_ARG, "connectivity entry out of range");
+      ctx->conn0[i] = (int32_t)(conn[i] - 1);
+    }
+    // coordinates as an FP32 hi/lo pair so element-local differences keep FP64-level accuracy
+    std::vector<float> hi(nn * nd), lo(nn * nd);
+    for (int64_t i = 0; i < nn * nd; ++i) {
+      hi[i] = (float)coords[i];
+      lo[i] = (float)(coords[i] - (double)hi[i]);
+    }
+    upload(ctx, "conn", ctx->conn0);
+    upload(ctx, "xhi", hi);
+    upload(ctx, "xlo", lo);
+    ctx->has_mesh = true;
+    ctx->has_dofs = ctx->has_pattern = false;
+  });
+}
+
+int fecx_set_dofs(fecx_ctx* ctx, int nf, const int64_t* dirichlet, int64_t ndir) {
+  return guard(ctx, [&] {
+    require(ctx->has_mesh, FECX_ERR_STATE, "fecx_set_mesh has not been called");
+    require(nf >= 1 && nf <= 3, FECX_ERR_ARG, "nf must be 1..3");
+    require(ndir >= 0 && (ndir == 0 || dirichlet), FECX_ERR_ARG, "bad Dirichlet list");
+    const int64_t N = ctx->nn * nf;
+    ctx->nf = nf;
+    ctx->dirichlet.assign(dirichlet, dirichlet + ndir);
+    std::vector<char> isbc(N + 1, 0);
+    for (int64_t i = 0; i < ndir; ++i) {
+      require(dirichlet[i] >= 1 && dirichlet[i] <= N, FECX_ERR_ARG, "Dirichlet dof out of range");
+      require(i == 0 || dirichlet[i] > dirichlet[i - 1], FECX_ERR_ARG, "Dirichlet dofs must be sorted and unique");
+      isbc[dirichlet[i]] = 1;
+    }
+    ctx->unknown_dofs.clear();
+    for (int64_t d = 1; d <= N; ++d)
+      if (!isbc[d]) ctx->unknown_dofs.push_back(d);
+    ctx->node_free.assign(ctx->nn, 1);
+    for (int64_t n = 0; n < ctx->nn; ++n) {
+      int c = 0;
+      for (int i = 0; i < nf; ++i) c += isbc[n * nf + i + 1];
+      require(c == 0 || c == nf, FECX_ERR_UNSUPPORTED,
+              "each node's dofs must be all free or all Dirichlet when nf > 1");
+      ctx->node_free[n] = c == 0;
+    }
+    std::vector<int64_t> u0(ctx->unknown_dofs.size()), d0(ndir);
+    for (size_t i = 0; i < u0.size(); ++i) u0[i] = ctx->unknown_dofs[i] - 1;
+    for (int64_t i = 0; i < ndir; ++i) d0[i] = dirichlet[i] - 1;
+    upload(ctx, "unknown_idx", u0);
+    upload(ctx, "dirichlet_idx", d0);
+    ctx->has_dofs = true;
+    ctx->has_pattern = false;
+  });
+}
+
+int fecx_get_unknown_dofs(fecx_ctx* ctx, int64_t* n, int64_t* unknown) {
+  return guard(ctx, [&] {
+    require(ctx->has_dofs, FECX_ERR_STATE, "fecx_set_dofs has not been called");
+    if (n) *n = (int64_t)ctx->unknown_dofs.size();
+    if (unknown) std::memcpy(unknown, ctx->unknown_dofs.data(), ctx->unknown_dofs.size() * sizeof(int64_t));
+  });
+}
+
+int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
+  return guard(ctx, [&] {
+    require(ctx->has_dofs, FECX_ERR_STATE, "fecx_set_dofs has not been called");
+    ctx->pairs = fecx::make_pairs(ctx->nnpe);
+    require((ctx->ne * ctx->pairs.npairs) < (int64_t(1) << 31), FECX_ERR_ARG, "element storage exceeds 2^31 blocks");
+    fecx::build_host_pattern(ctx->hp, ctx->nn, ctx->ne, ctx->nnpe, ctx->conn0.data(), ctx->nf, ctx->node_free,
+                             ctx->pairs);
+    const int nf = ctx->nf;
+    ctx->nnz = ctx->hp.nnzb * nf * nf;
+    upload(ctx, "bptr", ctx->hp.bptr);
+    upload(ctx, "boff", ctx->hp.boff);
+    upload(ctx, "bent", ctx->hp.bent);
+    upload(ctx, "voff", ctx->hp.voff);
+    upload(ctx, "vent", ctx->hp.vent);
+    dev(ctx, "Rel", sizeof(float) * ctx->ne * ctx->nnpe * nf);
+    dev(ctx, "Kel", sizeof(float) * ctx->ne * ctx->pairs.npairs * nf * nf);
+    ctx->has_pattern = true;
+    if (nnz) *nnz = ctx->nnz;
+  });
+}
+
+int fecx_get_pattern(fecx_ctx* ctx, int64_t* rowptr, int64_t* colval, int base) {
+  return guard(ctx, [&] {
+    require(ctx->has_pattern, FECX_ERR_STATE, "fecx_build_pattern has not been called");
+    require(base == 0 || base == 1, FECX_ERR_ARG, "index base must be 0 or 1");
+    const auto& hp = ctx->hp;
+    const int nf = ctx->nf;
+    int64_t pos = 0;
+    for (int64_t r = 0; r < hp.nfree; ++r) {
+      const int64_t deg = hp.bptr[r + 1] - hp.bptr[r];
+      for (int i = 0; i < nf; ++i) {
+        if (rowptr) rowptr[r * nf + i] = pos + base;
+        for (int64_t s = 0; s < deg; ++s)
+          for (int k = 0; k < nf; ++k) {
+            if (colval) colval[pos] = (int64_t)nf * hp.bcol[hp.bptr[r] + s] + k + base;
+            ++pos;
+          }
+      }
+    }
+    if (rowptr) rowptr[hp.nfree * nf] = pos + base;
+  });
+}
+
+int fecx_get_coo_map(fecx_ctx* ctx, int64_t* map) {
+  return guard(ctx, [&] {
+    require(ctx->has_pattern, FECX_ERR_STATE, "fecx_build_pattern has not been called");
+    require(map != nullptr, FECX_ERR_ARG, "map is NULL");
+    const auto& hp = ctx->hp;
+    const int nf = ctx->nf, nnpe = ctx->nnpe, NDOF = nf * nnpe;
+    for (int64_t e = 0; e < ctx->ne; ++e) {
+      const int32_t* c = &ctx->conn0[e * nnpe];
+      for (int a = 0; a < nnpe; ++a) {
+        const int32_t ra = hp.free_rank[c[a]];
+        for (int i = 0; i < nf; ++i) {
+          int64_t* row = map + ((e * NDOF) + a * nf + i) * NDOF;
+          for (int b = 0; b < nnpe; ++b) {
+            const int32_t rb = hp.free_rank[c[b]];
+            int64_t base = -1;
+            int64_t deg = 0;
+            if (ra >= 0 && rb >= 0) {
+              const int32_t* b0 = hp.bcol.data() + hp.bptr[ra];
+              const int32_t* b1 = hp.bcol.data() + hp.bptr[ra + 1];
+              const int64_t s = std::lower_bound(b0, b1, rb) - b0;
+              deg = hp.bptr[ra + 1] - hp.bptr[ra];
+              base = (int64_t)nf * nf * hp.bptr[ra] + (int64_t)i * nf * deg + (int64_t)nf * s;
+            }
+            for (int k = 0; k < nf; ++k) row[b * nf + k] = base < 0 ? -1 : base + k;
+          }
+        }
+      }
+    }
+  });
+}
+
+int fecx_set_material(fecx_ctx* ctx, int kind, const double* props, int nprops) {
+  return guard(ctx, [&] {
+    require(ctx->has_mesh, FECX_ERR_STATE, "fecx_set_mesh has not been called");
+    require(nprops >= 0 && nprops <= 8 && (nprops == 0 || props), FECX_ERR_ARG, "bad properties");
+    require(kind >= FECX_POISSON && kind <= FECX_J2, FECX_ERR_UNSUPPORTED, "unsupported physics kind");
+    fecx::MatParams mp{};
+    mp.phys = kind;
+    for (int i = 0; i < nprops; ++i) mp.p[i] = (float)props[i];
+    if (kind == FECX_POISSON) {
+      require(nprops >= 2, FECX_ERR_ARG, "Poisson needs {source_kind, coefficient}");
+      mp.source_kind = (int)props[0];
+    }
+    if (kind == FECX_LINEAR_ELASTIC) require(nprops >= 3, FECX_ERR_ARG, "linear elastic needs {rho, K, G}");
+    if (kind == FECX_NEOHOOKEAN) require(nprops >= 2, FECX_ERR_ARG, "neo-Hookean needs {lambda, mu}");
+    if (kind == FECX_J2) require(nprops >= 4, FECX_ERR_ARG, "J2 needs {E, nu, sigma_y, H}");
+    ctx->mp = mp;
+    ctx->ns = kind == FECX_J2 ? 7 : 0;
+    if (ctx->ns && ctx->device >= 0) {
+      const size_t bytes = sizeof(float) * ctx->ns * ctx->nq * ctx->ne;
+      HIPCHK(hipMemset(dev(ctx, "state_old", bytes), 0, bytes));
+      HIPCHK(hipMemset(dev(ctx, "state_new", bytes), 0, bytes));
+    }
+    ctx->has_material = true;
+  });
+}
+
+int fecx_get_state_size(fecx_ctx* ctx, int* ns, int* nq) {
+  return guard(ctx, [&] {
+    if (ns) *ns = ctx->ns;
+    if (nq) *nq = ctx->nq;
+  });
+}
+
+int fecx_set_state_old(fecx_ctx* ctx, const float* st) {
+  return guard(ctx, [&] {
+    check_device(ctx);
+    require(ctx->ns > 0, FECX_ERR_STATE, "physics has no state");
+    HIPCHK(hipMemcpy(get<float>(ctx, "state_old"), st, sizeof(float) * ctx->ns * ctx->nq * ctx->ne,
+                     hipMemcpyHostToDevice));
+  });
+}
+
+int fecx_get_state_new(fecx_ctx* ctx, float* st) {
+  return guard(ctx, [&] {
+    check_device(ctx);
+    require(ctx->ns > 0, FECX_ERR_STATE, "physics has no state");
+    HIPCHK(hipMemcpy(st, get<float>(ctx, "state_new"), sizeof(float) * ctx->ns * ctx->nq * ctx->ne,
+                     hipMemcpyDeviceToHost));
+  });
+}
+
+int fecx_state_commit(fecx_ctx* ctx) {
+  return guard(ctx, [&] {
+    require(ctx->ns > 0, FECX_ERR_STATE, "physics has no state");
+    std::swap(ctx->buf["state_old"], ctx->buf["state_new"]);
+  });
+}
+
+int fecx_assemble_vector(fecx_ctx* ctx, const double* U, double* R) {
+  return guard(ctx, [&] {
+    require(R != nullptr, FECX_ERR_ARG, "R is NULL");
+    assemble(ctx, fecx::MODE_R, U, nullptr, R, nullptr);
+  });
+}
+
+int fecx_assemble_matrix(fecx_ctx* ctx, const double* U, double* vals) {
+  return guard(ctx, [&] {
+    require(vals != nullptr, FECX_ERR_ARG, "vals is NULL");
+    assemble(ctx, fecx::MODE_K, U, nullptr, nullptr, vals);
+  });
+}
+
+int fecx_assemble_vector_matrix(fecx_ctx* ctx, const double* U, double* R, double* vals) {
+  return guard(ctx, [&] {
+    require(R != nullptr && vals != nullptr, FECX_ERR_ARG, "R or vals is NULL");
+    assemble(ctx, fecx::MODE_R | fecx::MODE_K, U, nullptr, R, vals);
+  });
+}
+
+int fecx_assemble_matrix_action(fecx_ctx* ctx, const double* U, const double* V, double* KV) {
+  return guard(ctx, [&] {
+    require(V != nullptr && KV != nullptr, FECX_ERR_ARG, "V or KV is NULL");
+    assemble(ctx, fecx::MODE_ACTION_ASM, U, V, KV, nullptr);
+  });
+}
+
+int fecx_assemble_matrix_free_action(fecx_ctx* ctx, const double* U, const double* V, double* KV) {
+  return guard(ctx, [&] {
+    require(V != nullptr && KV != nullptr, FECX_ERR_ARG, "V or KV is NULL");
+    assemble(ctx, fecx::MODE_ACTION_MF, U, V, KV, nullptr);
+  });
+}
+
+int fecx_update_field_unknowns(fecx_ctx* ctx, const double* Uu, double* U) {
+  return guard(ctx, [&] {
+    check_device(ctx);
+    require(ctx->has_dofs, FECX_ERR_STATE, "fecx_set_dofs has not been called");
+    fecx::launch_scatter_unknowns((int64_t)ctx->unknown_dofs.size(), get<int64_t>(ctx, "unknown_idx"), Uu, U,
+                                  ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int fecx_update_field_dirichlet(fecx_ctx* ctx, const double* vals, double* U) {
+  return guard(ctx, [&] {
+    check_device(ctx);
+    require(ctx->has_dofs, FECX_ERR_STATE, "fecx_set_dofs has not been called");
+    fecx::launch_scatter_unknowns((int64_t)ctx->dirichlet.size(), get<int64_t>(ctx, "dirichlet_idx"), vals, U,
+                                  ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int fecx_extract_field_unknowns(fecx_ctx* ctx, const double* U, double* Uu) {
+  return guard(ctx, [&] {
+    check_device(ctx);
+    require(ctx->has_dofs, FECX_ERR_STATE, "fecx_set_dofs has not been called");
+    fecx::launch_gather_unknowns((int64_t)ctx->unknown_dofs.size(), get<int64_t>(ctx, "unknown_idx"), U, Uu,
+                                 ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int fecx_set_profiling(fecx_ctx* ctx, int on) {
+  return guard(ctx, [&] { ctx->profiling = on != 0; });
+}
+
+int fecx_profile_count(fecx_ctx* ctx, int* count) {
+  return guard(ctx, [&] { *count = (int)ctx->prof_names.size(); });
+}
+
+int fecx_profile_entry(fecx_ctx* ctx, int i, char* name, int name_len, int64_t* launches, double* total_ms) {
+  return guard(ctx, [&] {
+    require(i >= 0 && i < (int)ctx->prof_names.size(), FECX_ERR_ARG, "profile index out of range");
+    const std::string& n = ctx->prof_names[i];
+    if (name && name_len > 0) {
+      std::strncpy(name, n.c_str(), name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    const ProfEntry& pe = ctx->prof[n];
+    if (launches) *launches = pe.launches;
+    if (total_ms) *total_ms = pe.ms;
+  });
+}
+
+int fecx_profile_reset(fecx_ctx* ctx) {
+  return guard(ctx, [&] {
+    ctx->prof.clear();
+    ctx->prof_names.clear();
+  });
+}
+
+}  // extern "C"

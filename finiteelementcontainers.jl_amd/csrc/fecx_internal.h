@@ -1,0 +1,145 @@
+// Internal declarations shared by the host C-ABI layer and the HIP kernels of libfecx.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/fecx.h"
+
+namespace fecx {
+
+// ------------------------------------------------------------------ element geometry
+// Lanes per element in the element kernels: hex8 and quad4 map one lane to each local node
+// and each quadrature point (8 and 4); tet10 uses 16 lanes (10 nodes, 4 quadrature points).
+struct ElemInfo {
+  int kind, nnpe, nd, nq, lanes, npairs;
+};
+ElemInfo elem_info(int kind);
+
+// Symmetric element-matrix storage: one NF x NF block per unordered node pair {a, b}. Lane a owns
+// the pairs (a, (a+d) mod nnpe) for d = 0..(nnpe-1)/2, plus d = nnpe/2 when nnpe is even and
+// a < nnpe/2, so every lane's blocks are contiguous in the element's storage (slot = lane-major).
+struct PairTable {
+  int nnpe = 0, npairs = 0;
+  int lane_off[17] = {0}, lane_cnt[16] = {0};
+  int pa[64] = {0}, pb[64] = {0};          // slot -> (a, b): the stored block is K_ab
+  int slot_of[16][16];                     // unordered pair -> slot
+  bool stored_ab[16][16];                  // true if slot_of[a][b] stores K_ab (else K_ba)
+};
+PairTable make_pairs(int nnpe);
+
+// ------------------------------------------------------------------ host pattern
+// Block-CSR view of the reduced CSR pattern: a "free node" has all NF dofs unknown. Free nodes are
+// ranked in node order; the unknown index of (free rank r, dof i) is NF*r + i, so every CSR row of
+// node r has NF*deg(r) columns made of deg(r) runs of NF consecutive columns.
+struct HostPattern {
+  int nf = 0, nnpe = 0;
+  int64_t nn = 0, ne = 0, nfree = 0, nnzb = 0, ncontrib = 0;
+  std::vector<int32_t> free_rank;   // nn: rank or -1
+  std::vector<int32_t> free_node;   // nfree: node id (0-based)
+  std::vector<int64_t> bptr;        // nfree + 1 : block row pointer
+  std::vector<int32_t> bcol;        // nnzb: free rank of the column node, ascending per row
+  std::vector<int64_t> boff;        // nnzb + 1 : contributions of each block
+  std::vector<uint32_t> bent;       // ncontrib: (e * npairs + slot) | (transposed << 31)
+  std::vector<int32_t> voff;        // nn + 1 : node -> incident (element, local node)
+  std::vector<int32_t> vent;        // ne * nnpe : e * nnpe + a, ascending e
+};
+
+// conn0: 0-based node ids, ne x nnpe row-major (element-major); node_free: nn flags.
+void build_host_pattern(HostPattern& hp, int64_t nn, int64_t ne, int nnpe, const int32_t* conn0, int nf,
+                        const std::vector<char>& node_free, const PairTable& pt);
+
+// ------------------------------------------------------------------ kernels
+struct MatParams {
+  int phys;
+  int source_kind;
+  float p[8];
+};
+
+enum Mode { MODE_R = 1, MODE_K = 2, MODE_ACTION_MF = 4, MODE_ACTION_ASM = 8 };
+
+struct ElementArgs {
+  int64_t ne;
+  const int32_t* conn;      // ne x nnpe (0-based)
+  const float* xhi;         // nd x nn
+  const float* xlo;         // nd x nn
+  const double* U;          // nf x nn
+  const double* V;          // nf x nn (actions)
+  const float* st_old;      // ns x nq x ne
+  float* st_new;
+  float* Rel;               // ne x ndof   (residual or action output)
+  float* Kel;               // ne x npairs x nf*nf
+  int* flag;                // error flag (bit 1: bad Jacobian / NaN)
+  MatParams mp;
+};
+
+// returns 0, or -1 if the (kind, nf, phys) combination has no kernel
+int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t s);
+
+struct ReduceArgs {
+  int64_t nfree, nn;
+  const int64_t* bptr;
+  const int64_t* boff;
+  const uint32_t* bent;
+  const int32_t* voff;
+  const int32_t* vent;
+  const float* Kel;
+  const float* Rel;
+  double* vals;
+  double* R;
+};
+int launch_reduce_matrix(int nf, const ReduceArgs& a, hipStream_t s);
+int launch_reduce_vector(int nf, const ReduceArgs& a, hipStream_t s);
+
+// DOF plumbing
+void launch_scatter_unknowns(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s);
+void launch_gather_unknowns(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s);
+
+}  // namespace fecx
+
+// ------------------------------------------------------------------ context
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct ProfEntry {
+  int64_t launches = 0;
+  double ms = 0;
+};
+
+struct fecx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // mesh
+  bool has_mesh = false;
+  int kind = -1, nd = 0, nnpe = 0, nq = 0;
+  int64_t nn = 0, ne = 0;
+  std::vector<int32_t> conn0;  // ne x nnpe, 0-based
+  // dofs
+  bool has_dofs = false;
+  int nf = 0;
+  std::vector<int64_t> dirichlet;     // 1-based
+  std::vector<int64_t> unknown_dofs;  // 1-based
+  std::vector<char> node_free;
+  // pattern
+  bool has_pattern = false;
+  fecx::PairTable pairs;
+  fecx::HostPattern hp;
+  int64_t nnz = 0;
+  // material
+  bool has_material = false;
+  fecx::MatParams mp{};
+  int ns = 0;
+  // device buffers
+  std::map<std::string, DevBuf> buf;
+  // profiling
+  bool profiling = false;
+  std::vector<std::string> prof_names;
+  std::map<std::string, ProfEntry> prof;
+};

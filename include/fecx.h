@@ -1,0 +1,185 @@
+/*
+ * fecx.h -- C ABI of the MI355X element-assembly library (libfecx.so).
+ *
+ * This is the drop-in boundary for FiniteElementContainers.jl's element-assembly hot path.
+ * Every entry point below replaces one reference function (cited per declaration,
+ * paths relative to the FiniteElementContainers.jl repository). The Julia side keeps the
+ * reference's own entry points (assemble_vector!, assemble_stiffness!/assemble_matrix!,
+ * assemble_matrix_action!, assemble_matrix_free_action!, residual, stiffness, hvp) and
+ * forwards to these functions through ccall.
+ *
+ * Conventions
+ *  - Return value: 0 on success, a negative FECX_ERR_* code otherwise; the message is
+ *    available from fecx_last_error(ctx) until the next call on that context.
+ *  - Setup calls (fecx_set_*, fecx_get_*) take HOST pointers, borrowed for the call.
+ *  - Assembly calls take DEVICE pointers (hipDeviceptr_t cast to the element type) that
+ *    live on the context's device, e.g. a Julia ROCArray or a torch tensor; no copies.
+ *  - Indices crossing the boundary are 1-based int64 like the reference, unless stated.
+ *  - Nodal fields are NF x NN column-major (dof = NF*(node-1)+d, src/Fields.jl:36-40).
+ *  - Calls on one context are not re-entrant. Assembly calls are enqueued on the
+ *    context's stream (fecx_set_stream) and the call returns after the stream is
+ *    synchronised, so results can be read at once (the reference's callers read them
+ *    immediately, src/Solvers.jl:128-153).
+ */
+#ifndef FECX_H
+#define FECX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fecx_ctx fecx_ctx;
+
+/* error codes */
+#define FECX_OK 0
+#define FECX_ERR_ARG -1         /* bad argument or size mismatch (the reference's @assert) */
+#define FECX_ERR_STATE -2       /* call out of order (e.g. assembly before fecx_build_pattern) */
+#define FECX_ERR_HIP -3         /* HIP runtime failure */
+#define FECX_ERR_UNSUPPORTED -4 /* element/physics/BC combination not supported (UnsupportedSparseMatrixType analogue) */
+#define FECX_ERR_NUMERIC -5     /* non-positive Jacobian determinant or NaN met in an element kernel */
+
+/* element kinds (ReferenceFiniteElements Quad4 / Hex8 / Tet10) */
+#define FECX_QUAD4 0
+#define FECX_HEX8 1
+#define FECX_TET10 2
+
+/* physics kinds (the reference's physics callbacks cannot cross a C ABI; each kind names one)
+ * POISSON         test/poisson/TestPoissonCommon.jl:75-127; props = {source_kind, coefficient}
+ *                 source_kind 0: f = coefficient; 1: f = coefficient * nd*pi^2 * prod_d sin(pi x_d)
+ * LINEAR_ELASTIC  test/mechanics/TestMechanicsCommon.jl:14-20,140-222; props = {rho, K, G}
+ * NEOHOOKEAN      compressible neo-Hookean; props = {lambda, mu}
+ * J2              small-strain J2 plasticity, linear isotropic hardening; props = {E, nu, sigma_y, H};
+ *                 7 state variables per quadrature point
+ * J2_SURROGATE    J2 with the MLP stress/state surrogate and plain-model fallback (fecx_surrogate_*) */
+#define FECX_POISSON 0
+#define FECX_LINEAR_ELASTIC 1
+#define FECX_NEOHOOKEAN 2
+#define FECX_J2 3
+#define FECX_J2_SURROGATE 4
+
+/* ---------------------------------------------------------------- context */
+
+/* Creates a context on HIP device `device`. Replaces the `to_backend(ROCBackend, ...)` adaption
+ * of the assembler and parameters (ext/AMDGPUExt.jl:8-14). */
+int fecx_create(fecx_ctx** ctx, int device);
+
+/* Frees the context and every device buffer it owns. */
+int fecx_destroy(fecx_ctx* ctx);
+
+/* Message of the last failed call on ctx (never NULL). */
+const char* fecx_last_error(const fecx_ctx* ctx);
+
+/* Stream for every kernel this context launches (a hipStream_t; NULL = the null stream).
+ * Replaces the task-local stream KernelAbstractions launches on (src/Utils.jl:157-158). */
+int fecx_set_stream(fecx_ctx* ctx, void* hip_stream);
+
+/* ---------------------------------------------------------------- setup (host pointers) */
+
+/* One element block: coords is nd x nn column-major FP64 (the mesh's nodal_coords H1Field,
+ * src/meshes/StructuredMesh.jl:91), conn is nnpe x ne column-major 1-based (Connectivity.data
+ * of that block, src/Fields.jl:129-170). Replaces FunctionSpace's connectivity/coordinate
+ * storage (src/FunctionSpaces.jl:158-237). */
+int fecx_set_mesh(fecx_ctx* ctx, int elem_kind, int nd, int64_t nn, const double* coords, int64_t ne,
+                  const int64_t* conn);
+
+/* nf fields per node and the Dirichlet DOF list, sorted and unique, 1-based (the reference's
+ * dirichlet_dofs(bcs), src/bcs/DirichletBCs.jl:396-398). Performs update_dofs!
+ * (src/DofManagers.jl:227-298) without periodic pairs. For nf > 1 each node's dofs must be
+ * all free or all constrained (FECX_ERR_UNSUPPORTED otherwise). */
+int fecx_set_dofs(fecx_ctx* ctx, int nf, const int64_t* dirichlet_dofs, int64_t n_dirichlet);
+
+/* n_unknowns and, if unknown_dofs != NULL, the unknown DOF ids (1-based, ascending):
+ * DofManager.unknown_dofs after update_dofs! (src/DofManagers.jl:262-263). */
+int fecx_get_unknown_dofs(fecx_ctx* ctx, int64_t* n_unknowns, int64_t* unknown_dofs);
+
+/* Builds the reduced CSR sparsity pattern and the COO->CSR index maps on the host and uploads
+ * them. Replaces SparseMatrixPattern(dof) + _update_dofs!(pattern, ...) + the COO->CSC->CSR
+ * conversion of stiffness(asm) (src/assemblers/SparsityPatterns.jl:53-117,160-231,301-329).
+ * *nnz receives the number of CSR entries. */
+int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz);
+
+/* CSR pattern (n_unknowns+1 row pointers, nnz column indices) with the given index base
+ * (1 = SparseMatrixCSR(csc) convention, 0 = C). Bit-exact with SparseArrays.sparse! followed by
+ * SparseMatrixCSR (src/assemblers/SparsityPatterns.jl:301-308,326-329). */
+int fecx_get_pattern(fecx_ctx* ctx, int64_t* rowptr, int64_t* colval, int index_base);
+
+/* The COO->CSR index map: for every element-storage slot (ne * (nf*nnpe)^2 entries, element-major,
+ * then row i, then column j, as SparsityPatterns.jl:72-84 enumerates them) the 0-based CSR
+ * position the slot sums into, or -1 when the slot's row or column is a Dirichlet DOF
+ * (dropped by _update_dofs!, SparsityPatterns.jl:195-213). */
+int fecx_get_coo_map(fecx_ctx* ctx, int64_t* map);
+
+/* Physics kind and properties for the block; also (re)allocates the quadrature state
+ * (NS x NQ x NE, FP32) set to zero when the kind carries state (src/Parameters.jl:1-23). */
+int fecx_set_material(fecx_ctx* ctx, int kind, const double* props, int nprops);
+
+/* Number of state variables per quadrature point and quadrature points per element. */
+int fecx_get_state_size(fecx_ctx* ctx, int* ns, int* nq);
+
+/* Copies state_old from / state_new to host FP32 arrays of NS*NQ*NE (StateVariableField layout,
+ * src/Fields.jl:401-455). */
+int fecx_set_state_old(fecx_ctx* ctx, const float* state);
+int fecx_get_state_new(fecx_ctx* ctx, float* state);
+
+/* state_old <- state_new. The reference has no state swap (the caller owns it); this is it. */
+int fecx_state_commit(fecx_ctx* ctx);
+
+/* ---------------------------------------------------------------- assembly (device pointers) */
+
+/* Residual into the full NF x NN field R (zeroed first). U is the full field with Dirichlet
+ * values and unknowns already in place. Replaces assemble_vector!(storage, pattern, dof, func,
+ * Uu, p) after _update_for_assembly! (src/assemblers/Vector.jl:25-74, Assemblers.jl:402-432,72-87).
+ * Writes state_new for stateful physics. */
+int fecx_assemble_vector(fecx_ctx* ctx, const double* U, double* R);
+
+/* CSR values (nnz, FP64) of the Jacobian at U. Replaces assemble_matrix! followed by
+ * stiffness(asm) for sparse_matrix_type = :csr (src/assemblers/Matrix.jl:35-75,
+ * Assemblers.jl:376-388, SparsityPatterns.jl:301-329). */
+int fecx_assemble_matrix(fecx_ctx* ctx, const double* U, double* vals);
+
+/* Residual and Jacobian in one pass over the elements (one residual + one stiffness call of
+ * IterativeLinearSolver.solve!, src/Solvers.jl:133-141). */
+int fecx_assemble_vector_matrix(fecx_ctx* ctx, const double* U, double* R, double* vals);
+
+/* Assembled Jacobian action into the full field KV: per element K_el is built and multiplied by
+ * v_el. Replaces assemble_matrix_action! (src/assemblers/MatrixAction.jl:154-238). */
+int fecx_assemble_matrix_action(fecx_ctx* ctx, const double* U, const double* V, double* KV);
+
+/* Matrix-free Jacobian action into the full field KV (per quadrature point K_q v).
+ * Replaces assemble_matrix_free_action! / _full! (src/assemblers/MatrixAction.jl:9-77,99-149). */
+int fecx_assemble_matrix_free_action(fecx_ctx* ctx, const double* U, const double* V, double* KV);
+
+/* U[unknown_dofs] = Uu: update_field_unknowns! (src/DofManagers.jl:328-411). */
+int fecx_update_field_unknowns(fecx_ctx* ctx, const double* Uu, double* U);
+
+/* U[dirichlet_dofs] = vals (n_dirichlet values, in the order given to fecx_set_dofs):
+ * update_field_dirichlet_bcs! (src/bcs/DirichletBCs.jl:411-418). */
+int fecx_update_field_dirichlet(fecx_ctx* ctx, const double* vals, double* U);
+
+/* Uu = U[unknown_dofs]: extract_field_unknowns! (src/DofManagers.jl:203-213). */
+int fecx_extract_field_unknowns(fecx_ctx* ctx, const double* U, double* Uu);
+
+/* ---------------------------------------------------------------- instrumentation */
+
+/* When on, every kernel launch is bracketed by HIP events on the context's stream and its
+ * duration accumulated per kernel name (the reference's only instrumentation is TimerOutputs
+ * sections, src/Solvers.jl:133-152). */
+int fecx_set_profiling(fecx_ctx* ctx, int on);
+
+/* Number of distinct kernels profiled so far. */
+int fecx_profile_count(fecx_ctx* ctx, int* count);
+
+/* Entry i: kernel name (copied into name, NUL-terminated, at most name_len bytes), number of
+ * launches and total milliseconds. */
+int fecx_profile_entry(fecx_ctx* ctx, int i, char* name, int name_len, int64_t* launches, double* total_ms);
+
+/* Clears the accumulated profile. */
+int fecx_profile_reset(fecx_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FECX_H */

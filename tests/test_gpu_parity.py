@@ -1,0 +1,148 @@
+"""GPU parity: residual, CSR Jacobian, assembled and matrix-free actions from libfecx.so (FP32 element kernels,
+FP64 reductions) against the FP64 oracle on the same seeded meshes and fields.
+
+Tolerances (relative, 2-norm over all entries): Jacobian values 2e-6, actions 2e-6, residual 2e-5 -- FP32 element
+arithmetic (unit roundoff 6e-8) summed over <= 8 elements x 8 quadrature points; the residual bound is looser because
+interior residuals of smooth fields are sums of cancelling element contributions. The sparsity pattern is compared
+bit-exactly."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"R": 2e-5, "K": 2e-6, "KV": 2e-6}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _kind(orc, mesh):
+    return {"QUAD4": orc.QUAD4, "HEX8": orc.HEX8, "TET10": orc.TET10}[mesh.element_type]
+
+
+def _setup(fec, wl, gpu):
+    mesh = wl.mesh
+    V = fec.FunctionSpace(mesh, "H1Field", "Lagrange")
+    var = fec.ScalarFunction(V, "u") if wl.nf == 1 else fec.VectorFunction(V, "displ")
+    asm = fec.SparseMatrixAssembler(var, sparse_matrix_type="csr", device=gpu.index or 0)
+    bcs = []
+    if len(wl.dirichlet_dofs):
+        # Dirichlet on the given dofs with value 0 (all components of the listed nodes)
+        nodes = np.unique((wl.dirichlet_dofs - 1) // wl.nf + 1)
+        mesh.nodeset_nodes["_bc"] = nodes
+        bcs = [fec.DirichletBC(name, lambda X, t: 0.0, nset_name="_bc") for name in var.names]
+    p = fec.create_parameters(mesh, asm, wl.physics, dirichlet_bcs=bcs)
+    return asm, p
+
+
+def _oracle_all(orc, wl, U, Vv, unknown, d2u, st_old=None):
+    mesh = wl.mesh
+    k = _kind(orc, mesh)
+    props = wl.physics.props()
+    R = orc.assemble_residual(k, mesh.coords, mesh.conn, wl.nf, wl.physics.kind, props, U, st_old,
+                              None if st_old is None else np.zeros_like(st_old))
+    rp, cv, vals = orc.stiffness_csr(k, mesh.coords, mesh.conn, wl.nf, wl.physics.kind, props, U, d2u, len(unknown),
+                                     st_old)
+    KVa = orc.assemble_action(k, mesh.coords, mesh.conn, wl.nf, wl.physics.kind, props, U, Vv, st_old)
+    KVm = orc.assemble_action(k, mesh.coords, mesh.conn, wl.nf, wl.physics.kind, props, U, Vv, st_old,
+                              matrix_free=True)
+    return R[unknown - 1], (rp, cv, vals), KVa[unknown - 1], KVm[unknown - 1]
+
+
+CASES = ["poisson_quad4", "poisson_hex8", "poisson_tet10", "mechanics_hex8", "neohookean_hex8", "neohookean_tet10",
+         "j2_hex8"]
+
+
+def _workload(fec, name):
+    W = fec.workloads
+    if name == "poisson_quad4":
+        return W.poisson_quad4(12)
+    if name == "poisson_hex8":
+        return W.poisson_hex8(6)
+    if name == "poisson_tet10":
+        wl = W.neohookean_tet10(2)
+        return W.Workload("poisson_tet10", wl.mesh, fec.Poisson("sines", 1.0), 1,
+                          2 * fec.rng.uniform(0, 1, wl.mesh.num_nodes) - 1, fec.boundary_nodes(wl.mesh))
+    if name == "mechanics_hex8":
+        wl = W.neohookean_hex8(5)
+        return W.Workload("mechanics_hex8", wl.mesh, fec.Mechanics(), 3, 0.01 * wl.U)
+    if name == "neohookean_hex8":
+        return W.neohookean_hex8(6)
+    if name == "neohookean_tet10":
+        return W.neohookean_tet10(3)
+    if name == "j2_hex8":
+        return W.j2_hex8(4, dirichlet=True)
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_operators_match_oracle(fec, orc, gpu, name):
+    print(f"[gpu parity] {name}", flush=True)
+    wl = _workload(fec, name)
+    asm, p = _setup(fec, wl, gpu)
+    unknown = asm.dof.unknown_dofs
+    N = wl.nf * wl.mesh.num_nodes
+    _, d2u = orc.update_dofs(N, asm.dof.dirichlet_dofs)
+    U = wl.U.copy()
+    U[asm.dof.dirichlet_dofs - 1] = 0.0
+    Vv = np.zeros(N)
+    Vv[unknown - 1] = fec.rng.normal(3, 9, len(unknown))
+    st_old = None
+    if wl.physics.kind == orc.J2:
+        nq = 8
+        st_old = np.zeros((wl.mesh.num_elements, nq, 7))
+        g = fec.rng.normal(1, 11, st_old.size).reshape(st_old.shape)
+        st_old[..., :6] = 2e-3 * g[..., :6]
+        st_old[..., 6] = 2e-3 * np.abs(g[..., 6])
+        st_old = st_old.ravel()
+        asm.ctx.call("fecx_set_state_old", st_old.astype(np.float32).ctypes.data_as(__import__("ctypes").c_void_p))
+        st_old = st_old.astype(np.float32).astype(np.float64)
+    Uu = torch.from_numpy(U[unknown - 1]).to(gpu)
+    Vu = torch.from_numpy(Vv[unknown - 1]).to(gpu)
+    R_ref, (rp, cv, vals), KVa_ref, KVm_ref = _oracle_all(orc, wl, U, Vv, unknown, d2u, st_old)
+
+    fec.assemble_vector_(asm, fec.residual, Uu, p)
+    R = fec.residual(asm).cpu().numpy()
+    fec.assemble_stiffness_(asm, fec.stiffness, Uu, p)
+    K = fec.stiffness(asm)
+    Kvals = K.values.cpu().numpy().copy()
+    fec.assemble_matrix_action_(asm, fec.stiffness, Uu, Vu, p)
+    KVa = fec.hvp(asm, Vu).cpu().numpy().copy()
+    fec.assemble_matrix_free_action_(asm, fec.stiffness_action, Uu, Vu, p)
+    KVm = fec.hvp(asm, Vu).cpu().numpy().copy()
+
+    assert np.array_equal(K.rowptr.cpu().numpy() + 1, rp) and np.array_equal(K.colval.cpu().numpy() + 1, cv)
+    errs = {"R": rel(R, R_ref), "K": rel(Kvals, vals), "KVa": rel(KVa, KVa_ref), "KVm": rel(KVm, KVm_ref)}
+    print(f"[gpu parity] {name}: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    assert errs["R"] < TOL["R"] and errs["K"] < TOL["K"]
+    assert errs["KVa"] < TOL["KV"] and errs["KVm"] < TOL["KV"]
+
+    # one fused pass gives the same numbers as the two separate calls, bit for bit (no atomics anywhere)
+    fec.assemble_vector_and_stiffness_(asm, Uu, p)
+    assert np.array_equal(fec.residual(asm).cpu().numpy(), R)
+    assert np.array_equal(asm.stiffness_storage.cpu().numpy(), Kvals)
+
+    if st_old is not None:
+        ns = np.zeros(st_old.size, dtype=np.float32)
+        fec.assemble_vector_(asm, fec.residual, Uu, p)
+        asm.ctx.call("fecx_get_state_new", ns.ctypes.data_as(__import__("ctypes").c_void_p))
+        st_ref = np.zeros_like(st_old)
+        k = _kind(orc, wl.mesh)
+        orc.assemble_residual(k, wl.mesh.coords, wl.mesh.conn, 3, orc.J2, wl.physics.props(), U, st_old, st_ref)
+        assert np.abs(ns - st_ref).max() < 1e-6 * max(1.0, np.abs(st_ref).max()) + 1e-9
+        assert (st_ref[6::7] > st_old[6::7]).any(), "the J2 case should yield somewhere"
+
+
+def test_numeric_error_flag(fec, gpu):
+    """A folded element (negative Jacobian) is reported as FECX_ERR_NUMERIC, not silently assembled."""
+    wl = fec.workloads.neohookean_hex8(3)
+    mesh = wl.mesh
+    mesh.coords = mesh.coords.copy()
+    mesh.coords[21] += np.array([0.6, 0.6, 0.6])  # interior node pushed through its neighbours
+    asm, p = _setup(fec, wl, gpu)
+    Uu = torch.from_numpy(np.zeros(3 * mesh.num_nodes)).to(gpu)
+    with pytest.raises(fec.FECXError) as ei:
+        fec.assemble_vector_(asm, fec.residual, Uu, p)
+    assert ei.value.code == -5

@@ -1,0 +1,82 @@
+"""libfecx's host pattern builder (the product code path, on a host-only context) against the oracle: DOF numbering,
+CSR pattern and the COO->CSR index map must be bit-exact with the reference algorithm (SparsityPatterns.jl:53-117,
+160-231, 301-329 as restated in oracle/oracle.cpp)."""
+import re
+import os
+import ctypes
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _cases(fec):
+    quad = fec.workloads.poisson_quad4(6)
+    hexm = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (4, 5, 4))
+    b = fec.boundary_nodes(hexm)
+    hex_bc = np.sort(np.concatenate([3 * (b - 1) + d + 1 for d in range(3)]))
+    tet = fec.kuhn_tet10_mesh(2)
+    return {
+        "quad4_nf1_dirichlet": (quad.mesh, 1, quad.dirichlet_dofs),
+        "hex8_nf1": (hexm, 1, np.zeros(0, np.int64)),
+        "hex8_nf3": (hexm, 3, np.zeros(0, np.int64)),
+        "hex8_nf3_dirichlet": (hexm, 3, hex_bc),
+        "hex8_nf1_dirichlet_scattered": (hexm, 1, np.array([1, 7, 8, 33, 50, 51, 79], np.int64)),
+        "tet10_nf3": (tet.__class__(tet.coords, tet.conn), 3, np.zeros(0, np.int64)),
+        "tet10_nf1_dirichlet": (tet, 1, fec.boundary_nodes(tet)),
+    }
+
+
+@pytest.mark.parametrize("name", ["quad4_nf1_dirichlet", "hex8_nf1", "hex8_nf3", "hex8_nf3_dirichlet",
+                                  "hex8_nf1_dirichlet_scattered", "tet10_nf3", "tet10_nf1_dirichlet"])
+def test_pattern_and_map_bit_exact(fec, orc, name):
+    mesh, nf, dd = _cases(fec)[name]
+    hp = fec.HostPattern(mesh, nf, dd)
+    N = nf * mesh.num_nodes
+    unknown, d2u = orc.update_dofs(N, dd)
+    assert np.array_equal(hp.unknown_dofs, unknown)
+    Is, Js, slots, perm = orc.pattern(nf, mesh.conn, d2u)
+    rowptr, colval, _, cmap = orc.sparse_csr(len(unknown), Is, Js, want_map=True)
+    rp, cv = hp.pattern(base=1)
+    assert np.array_equal(rp, rowptr) and np.array_equal(cv, colval)
+    rp0, cv0 = hp.pattern(base=0)
+    assert np.array_equal(rp0, rowptr - 1) and np.array_equal(cv0, colval - 1)
+    full = -np.ones(mesh.conn.shape[0] * (mesh.conn.shape[1] * nf) ** 2, dtype=np.int64)
+    full[slots - 1] = cmap
+    assert np.array_equal(hp.coo_map(), full)
+
+
+def test_partial_node_dirichlet_rejected(fec):
+    mesh = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (3, 3, 3))
+    with pytest.raises(fec.FECXError) as ei:
+        fec.HostPattern(mesh, 3, np.array([1], np.int64))  # only the x dof of node 1
+    assert ei.value.code == -4
+
+
+def test_bad_inputs_raise(fec):
+    mesh = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (3, 3, 3))
+    with pytest.raises(fec.FECXError) as ei:
+        fec.HostPattern(mesh, 1, np.array([5, 3], np.int64))  # not sorted
+    assert ei.value.code == -1
+    with pytest.raises(fec.FECXError):
+        fec.HostPattern(mesh, 1, np.array([0], np.int64))  # out of range
+
+
+def test_library_exports_every_header_symbol(fec):
+    hdr = open(os.path.join(ROOT, "include", "fecx.h")).read()
+    declared = set(re.findall(r"\b(fecx_\w+)\s*\(", hdr))
+    assert declared, "no declarations parsed"
+    lib = ctypes.CDLL(fec.LIB_PATH)
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert declared == set(fec.exported_symbols()) | (declared - set(fec.exported_symbols()))
+    assert set(fec.exported_symbols()) <= declared
+
+
+def test_host_only_context_refuses_kernels(fec):
+    mesh = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (3, 3, 3))
+    hp = fec.HostPattern(mesh, 1)
+    with pytest.raises(fec.FECXError) as ei:
+        hp.ctx.call("fecx_extract_field_unknowns", None, None)
+    assert ei.value.code == -2
