@@ -4,10 +4,12 @@
   poisson_hex8(n=128)     unit cube, n^3 hex8, f = 1, unit conductivity, u ~ U[-1, 1] per node, no Dirichlet DOFs
   neohookean_hex8(n=96)   unit cube, n^3 hex8, lambda = 1, mu = 0.5,
                           u = 0.05 sum_{k=1..4} a_k sin(k pi x) sin(k pi y) sin(k pi z), a_k ~ N(0, I3)
+                          drawn with the first field seed whose continuous field has det(I + grad u) > 0.2
+                          everywhere (seed 33; seed 0 inverts 7% of the elements, where ln J is undefined)
   j2_hex8(n=64)           unit cube, n^3 hex8, E = 200, nu = 0.3, sigma_y = 0.25, H = 2; 20 load steps of a
                           piecewise-linear random-direction macroscopic strain path (|eps| <= 0.01)
-  neohookean_tet10(n=80)  unit cube, n^3 x 6 Kuhn tet10, interior vertices jittered +-0.2h, NH as above with
-                          amplitude 0.1
+  neohookean_tet10(n=80)  unit cube, n^3 x 6 Kuhn tet10, interior vertices jittered +-0.2h (seed 0), NH as above
+                          with amplitude 0.1 (field seed 1585 by the same rule; seed 0 inverts 28% of the elements)
 """
 from dataclasses import dataclass, field
 
@@ -33,7 +35,31 @@ class Workload:
         return self.mesh.num_elements
 
 
-def _sine_field(X, amp, seed, stream, kmax=4):
+FIELD_STREAM = 2
+# first seeds (scanning 0, 1, 2, ...) whose sine field keeps det(I + grad u) > 0.2 on a 41^3 grid of the cube
+FIELD_SEED = {0.05: 33, 0.1: 1585}
+
+
+def sine_field_coefficients(seed, kmax=4):
+    return rng.normal(seed, FIELD_STREAM, kmax * 3).reshape(kmax, 3)
+
+
+def sine_field_min_det(amp, seed, pts=41):
+    """min over a pts^3 grid of det(I + grad u) for the continuous sine field."""
+    a = sine_field_coefficients(seed)
+    g = np.linspace(0.0, 1.0, pts)
+    X, Y, Z = (v.ravel() for v in np.meshgrid(g, g, g, indexing="ij"))
+    H = np.zeros((X.size, 3, 3))
+    for k in range(1, 5):
+        kp = k * np.pi
+        sx, sy, sz = np.sin(kp * X), np.sin(kp * Y), np.sin(kp * Z)
+        cx, cy, cz = np.cos(kp * X), np.cos(kp * Y), np.cos(kp * Z)
+        grad = np.stack([kp * cx * sy * sz, kp * sx * cy * sz, kp * sx * sy * cz], axis=1)
+        H += amp * a[k - 1][None, :, None] * grad[:, None, :]
+    return float(np.linalg.det(np.eye(3)[None] + H).min())
+
+
+def _sine_field(X, amp, seed, stream=FIELD_STREAM, kmax=4):
     a = rng.normal(seed, stream, kmax * 3).reshape(kmax, 3)
     U = np.zeros_like(X)
     for k in range(1, kmax + 1):
@@ -54,14 +80,18 @@ def poisson_hex8(n=128, seed=0):
     return Workload("poisson_hex8", mesh, Poisson("constant", 1.0), 1, U)
 
 
-def neohookean_hex8(n=96, seed=0, amp=0.05):
+def neohookean_hex8(n=96, seed=None, amp=0.05):
+    seed = FIELD_SEED[amp] if seed is None else seed
     mesh = StructuredMesh("hex", (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (n + 1, n + 1, n + 1))
-    return Workload("neohookean_hex8", mesh, NeoHookean(1.0, 0.5), 3, _sine_field(mesh.coords, amp, seed, 2))
+    return Workload("neohookean_hex8", mesh, NeoHookean(1.0, 0.5), 3, _sine_field(mesh.coords, amp, seed),
+                    extra={"field_seed": seed})
 
 
-def neohookean_tet10(n=80, seed=0, amp=0.1):
-    mesh = kuhn_tet10_mesh(n, jitter=0.2, seed=seed)
-    return Workload("neohookean_tet10", mesh, NeoHookean(1.0, 0.5), 3, _sine_field(mesh.coords, amp, seed, 2))
+def neohookean_tet10(n=80, seed=None, amp=0.1, mesh_seed=0):
+    seed = FIELD_SEED[amp] if seed is None else seed
+    mesh = kuhn_tet10_mesh(n, jitter=0.2, seed=mesh_seed)
+    return Workload("neohookean_tet10", mesh, NeoHookean(1.0, 0.5), 3, _sine_field(mesh.coords, amp, seed),
+                    extra={"field_seed": seed})
 
 
 def j2_strain_path(seed=0, steps=20, max_norm=0.01, waypoints=4):

@@ -364,3 +364,17 @@ def test_poisson_gold(fec, orc):
     np.testing.assert_allclose(mesh.coords[idx, 0], g["x"], atol=1e-14)
     np.testing.assert_allclose(mesh.coords[idx, 1], g["y"], atol=1e-14)
     assert np.abs(u[idx] - g["u"]).max() < 1e-10
+
+
+@pytest.mark.parametrize("amp", [0.05, 0.1])
+def test_field_seed_rule(fec, amp):
+    """The neo-Hookean field seeds are the first seeds whose sine field keeps det(I + grad u) > 0.2 (checked here on
+    a coarse grid for the seeds below the chosen one, on the full grid for the chosen one)."""
+    W = fec.workloads
+    s = W.FIELD_SEED[amp]
+    assert W.sine_field_min_det(amp, s) > 0.2
+    for t in range(0, s, max(1, s // 40)):
+        a = W.sine_field_coefficients(t)
+        if np.abs(a[3]).max() * amp * 4 * np.pi > 0.9:
+            continue  # the k = 4 term alone folds the field
+        assert W.sine_field_min_det(amp, t, pts=41) <= 0.2, t
