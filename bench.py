@@ -50,21 +50,22 @@ def build_assembler(fec, wl, device=0):
     return asm, p
 
 
-def kernel_bytes(wl, asm):
+def kernel_bytes(wl, asm, with_r=True, with_k=True):
     """Algorithmic HBM bytes per launch of each kernel of the two-phase assembly (compulsory traffic)."""
     ne, nnpe = wl.mesh.conn.shape
     nn, nf = wl.mesh.num_nodes, wl.nf
     nd = wl.mesh.coords.shape[1]
     ndof = nnpe * nf
     npairs = nnpe * (nnpe + 1) // 2
-    nnzb = asm.nnz // (nf * nf)
     nfree = len(asm.dof.unknown_dofs) // nf
-    ncontrib = ne * nnpe * nnpe  # no Dirichlet DOFs in the benchmark workloads
+    vstride = (nnpe + 3) // 4 * 4
     state = 2 * 4 * 7 * 8 * ne if wl.physics.kind == 3 else 0
+    kel = 4 * npairs * nf * nf * ne if with_k else 0
+    rel_ = 4 * ndof * ne if with_r else 0
     return {
-        "element": 4 * nnpe * ne + 2 * 4 * nd * nn + 8 * nf * nn + 4 * ndof * ne + 4 * npairs * nf * nf * ne + state,
-        "reduce_matrix": 8 * (nfree + 1) + 8 * (nnzb + 1) + 4 * ncontrib + 4 * npairs * nf * nf * ne + 8 * asm.nnz,
-        "reduce_vector": 4 * (nn + 1) + 4 * ne * nnpe + 4 * ndof * ne + 8 * nf * nn,
+        "element": 4 * nnpe * ne + 2 * 4 * nd * nn + 8 * nf * nn + rel_ + kel + state,
+        "reduce": 8 * (nn + 1) + 8 * (nfree + 1) + 4 * nn + (4 + vstride) * ne * nnpe + kel + rel_
+        + (8 * asm.nnz if with_k else 0) + (8 * nf * nn if with_r else 0),
     }
 
 
@@ -115,7 +116,19 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="elements per direction (default: the config's size)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the sample parity/CPU legs (profiling runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": n * 8, "calibrate_copy_bytes_written": n * 8}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU assembly is a design in "
               "DESIGN.md, not a measured path", file=sys.stderr)

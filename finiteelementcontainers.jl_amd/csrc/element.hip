@@ -1,16 +1,19 @@
 // Element kernels (phase 1 of assembly): FP32 element math, element-private FP32 outputs.
 //
 // Reference: the generic element loop _assemble_block! (src/assemblers/Assemblers.jl:402-432) runs one
-// work-item per element and stores K_el into element-private COO slots (Assemblers.jl:109-124). Here one
-// wavefront processes 64/G elements with G lanes per element:
+// work-item per element and stores K_el into element-private COO slots (Assemblers.jl:109-124). Here a
+// 256-thread work-group processes 256/G elements with G lanes per element:
 //   gather   lane a  (a < nnpe) : connectivity, coordinates and nodal values of local node a; element-local
 //                                 differences x_a - x_0 (hi/lo FP32 coordinate split) and u_a - u_0
-//   qp       lane q  (q < nq)   : isoparametric map, shape gradients, field gradient and material point
-//                                 (stress, symmetric tangent) of quadrature point q, staged in LDS
+//   qp       lane q  (q < nq)   : isoparametric map, shape gradients, field gradient and material point of
+//                                 quadrature point q: stress and a compact form of the tangent, staged in LDS
 //   node     lane a  (a < nnpe) : residual / action rows of node a, and the node-pair blocks K_ab it owns
 //                                 (symmetric storage, one NF x NF block per unordered pair)
-// Outputs are FP32 element-private slots; the FP64 sums into the residual field and the CSR values happen
-// in the reduction kernels (reduce.hip), through the precomputed inverse COO->CSR map.
+// Each material contracts its tangent with the shape gradients in closed form (K_ab = sum_q w g_a . A . g_b
+// written through a few per-node vectors), so the LDS holds O(10) floats of tangent data per quadrature point
+// instead of the 45 of a dense symmetric 9 x 9 tangent.
+// Outputs are FP32 element-private slots in element storage order; the FP64 sums into the residual field and
+// the CSR values happen in the reduction kernel (reduce.hip), through the precomputed inverse COO->CSR map.
 #include "fecx_internal.h"
 
 namespace fecx {
@@ -33,144 +36,233 @@ struct ET<FECX_TET10> {
 __host__ __device__ constexpr int lane_cnt(int nnpe, int a) {
   return a >= nnpe ? 0 : (nnpe - 1) / 2 + 1 + ((nnpe % 2 == 0 && a < nnpe / 2) ? 1 : 0);
 }
-__host__ __device__ constexpr int lane_off(int nnpe, int a) {
-  return a * ((nnpe - 1) / 2 + 1) + ((nnpe % 2 == 0) ? (a < nnpe / 2 ? a : nnpe / 2) : 0);
-}
 
 // ------------------------------------------------------------------ reference elements (FP32)
 template <int KIND>
-__device__ __forceinline__ float qweight(int q) {
+__device__ __forceinline__ float qweight() {
   if constexpr (KIND == FECX_TET10) return 1.0f / 24.0f;
   else return 1.0f;
 }
 
-// N[a], dN[a][j] at quadrature point q. Gauss-Legendre 2 points per direction for quad4/hex8 (xi fastest),
-// the 4-point degree-2 rule (a=(5-sqrt5)/20, b=(5+3sqrt5)/20) for tet10.
+// Quadrature point q in reference coordinates: Gauss-Legendre 2 points per direction for quad4/hex8 (xi fastest),
+// the 4-point degree-2 rule (a=(5-sqrt5)/20, b=(5+3sqrt5)/20, weights 1/24) for tet10.
 template <int KIND>
-__device__ __forceinline__ void shape_at(int q, float* N, float (*dN)[ET<KIND>::ND]) {
+__device__ __forceinline__ void qpoint(int q, float* xi) {
   constexpr float g = 0.57735026918962576f;
   if constexpr (KIND == FECX_QUAD4) {
-    const float xi = (q & 1) ? g : -g, eta = (q & 2) ? g : -g;
-    const float s[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      float fx = 1 + s[a][0] * xi, fy = 1 + s[a][1] * eta;
-      N[a] = 0.25f * fx * fy;
-      dN[a][0] = 0.25f * s[a][0] * fy;
-      dN[a][1] = 0.25f * fx * s[a][1];
-    }
+    xi[0] = (q & 1) ? g : -g; xi[1] = (q & 2) ? g : -g;
   } else if constexpr (KIND == FECX_HEX8) {
-    const float xi = (q & 1) ? g : -g, eta = (q & 2) ? g : -g, zeta = (q & 4) ? g : -g;
-    const float s[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
-                           {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      float fx = 1 + s[a][0] * xi, fy = 1 + s[a][1] * eta, fz = 1 + s[a][2] * zeta;
-      N[a] = 0.125f * fx * fy * fz;
-      dN[a][0] = 0.125f * s[a][0] * fy * fz;
-      dN[a][1] = 0.125f * fx * s[a][1] * fz;
-      dN[a][2] = 0.125f * fx * fy * s[a][2];
-    }
+    xi[0] = (q & 1) ? g : -g; xi[1] = (q & 2) ? g : -g; xi[2] = (q & 4) ? g : -g;
   } else {
     constexpr float qa = 0.13819660112501051f, qb = 0.58541019662496845f;
-    float p[3] = {qa, qa, qa};
-    if (q >= 1) p[q - 1] = qb;
-    const float L[4] = {1.0f - p[0] - p[1] - p[2], p[0], p[1], p[2]};
-    const float dL[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    xi[0] = q == 1 ? qb : qa; xi[1] = q == 2 ? qb : qa; xi[2] = q == 3 ? qb : qa;
+  }
+}
+
+// N_a and dN_a/dxi of one local node a (quad4/hex8 in the structured mesh's local order, tet10 in Exodus TETRA10
+// order: vertices, then edges 12, 23, 31, 14, 24, 34).
+template <int KIND>
+__device__ __forceinline__ void shape_node(int a, const float* xi, float& N, float* dN) {
+  if constexpr (KIND == FECX_QUAD4) {
+    const float sx = ((a & 3) == 1 || (a & 3) == 2) ? 1.0f : -1.0f, sy = (a & 2) ? 1.0f : -1.0f;
+    const float fx = 1 + sx * xi[0], fy = 1 + sy * xi[1];
+    N = 0.25f * fx * fy;
+    dN[0] = 0.25f * sx * fy;
+    dN[1] = 0.25f * fx * sy;
+  } else if constexpr (KIND == FECX_HEX8) {
+    const float sx = ((a & 3) == 1 || (a & 3) == 2) ? 1.0f : -1.0f, sy = (a & 2) ? 1.0f : -1.0f;
+    const float sz = (a & 4) ? 1.0f : -1.0f;
+    const float fx = 1 + sx * xi[0], fy = 1 + sy * xi[1], fz = 1 + sz * xi[2];
+    N = 0.125f * fx * fy * fz;
+    dN[0] = 0.125f * sx * fy * fz;
+    dN[1] = 0.125f * fx * sy * fz;
+    dN[2] = 0.125f * fx * fy * sz;
+  } else {
+    // barycentric L0 = 1 - x - y - z, L1 = x, L2 = y, L3 = z
+    auto L = [&](int v) { return v == 0 ? 1.0f - xi[0] - xi[1] - xi[2] : xi[v - 1]; };
+    auto dL = [&](int v, int j) { return v == 0 ? -1.0f : (v - 1 == j ? 1.0f : 0.0f); };
+    if (a < 4) {
+      const float l = L(a);
+      N = l * (2 * l - 1);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      N[a] = L[a] * (2 * L[a] - 1);
+      for (int j = 0; j < 3; ++j) dN[j] = (4 * l - 1) * dL(a, j);
+    } else {
+      // edges 4:(0,1) 5:(1,2) 6:(2,0) 7:(0,3) 8:(1,3) 9:(2,3)
+      const int m = a - 4;
+      const int p = m < 3 ? m : m - 3;
+      const int r = m < 3 ? (m + 1) % 3 : 3;
+      const float lp = L(p), lr = L(r);
+      N = 4 * lp * lr;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) dN[a][j] = (4 * L[a] - 1) * dL[a][j];
-    }
-    const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      const int P = ed[m][0], Q = ed[m][1];
-      N[4 + m] = 4 * L[P] * L[Q];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) dN[4 + m][j] = 4 * (dL[P][j] * L[Q] + L[P] * dL[Q][j]);
+      for (int j = 0; j < 3; ++j) dN[j] = 4 * (dL(p, j) * lr + lp * dL(r, j));
     }
   }
 }
 
 // ------------------------------------------------------------------ materials (FP32)
-__device__ __forceinline__ int asym_idx(int r, int c) {
-  // upper triangle of a 9 x 9 symmetric matrix, row-major
-  return r <= c ? 8 * r - (r * (r - 1)) / 2 + c : 8 * c - (c * (c - 1)) / 2 + r;
-}
+// Each material provides:
+//   NC                         floats of compact tangent data per quadrature point
+//   eval(mp, H, so, sn, P, C)  stress P (NF x ND) and tangent data C from the field gradient H (and state)
+//   prep(mp, C, g, v)          per-node vector v (NV floats) from the shape gradient g: K_ab depends on the
+//                              pair only through v_a, v_b
+//   pair(mp, C, va, vb, w, K)  K += w * (g_a . A . g_b) as an NF x NF block
+//   dP(mp, C, Gv, out)         out = A : Gv (matrix-free action)
+template <int PHYS>
+struct Mat;
 
-// Neo-Hookean, psi = mu/2 (I1 - 3) - mu ln J + lambda/2 (ln J)^2, written in the displacement gradient H so
-// that no term cancels for small H: J - 1 = tr H + (tr^2 H - tr H^2)/2 + det H, ln J = log1p(J - 1),
-// P = mu H + mu H^T F^-T + lambda ln J F^-T.
-__device__ __forceinline__ void neohookean(float lam, float mu, const float* H, float* P, float* A, bool& bad) {
-  const float tr = H[0] + H[4] + H[8];
-  const float trH2 = H[0] * H[0] + H[4] * H[4] + H[8] * H[8] + 2.0f * (H[1] * H[3] + H[2] * H[6] + H[5] * H[7]);
-  const float detH = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
-                     H[2] * (H[3] * H[7] - H[4] * H[6]);
-  const float Jm1 = tr + 0.5f * (tr * tr - trH2) + detH;
-  const float J = 1.0f + Jm1;
-  if (!(J > 0.0f)) bad = true;
-  const float lnJ = log1pf(Jm1);
-  const float F[9] = {1 + H[0], H[1], H[2], H[3], 1 + H[4], H[5], H[6], H[7], 1 + H[8]};
-  // cofactor(F) = J F^-T
-  float C[9];
-  C[0] = F[4] * F[8] - F[5] * F[7]; C[1] = F[5] * F[6] - F[3] * F[8]; C[2] = F[3] * F[7] - F[4] * F[6];
-  C[3] = F[2] * F[7] - F[1] * F[8]; C[4] = F[0] * F[8] - F[2] * F[6]; C[5] = F[1] * F[6] - F[0] * F[7];
-  C[6] = F[1] * F[5] - F[2] * F[4]; C[7] = F[2] * F[3] - F[0] * F[5]; C[8] = F[0] * F[4] - F[1] * F[3];
-  const float iJ = 1.0f / J;
-  float FiT[9];
+// -div(grad u) = f (test/poisson/TestPoissonCommon.jl:75-127): P = grad u, A = I
+template <>
+struct Mat<FECX_POISSON> {
+  static constexpr int NF = 1, NC = 1, NV = 3;
+  template <int ND>
+  __device__ static void eval(const MatParams&, const float* H, const float*, float*, float* P, float*, bool&) {
 #pragma unroll
-  for (int i = 0; i < 9; ++i) FiT[i] = C[i] * iJ;
-  if (P) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        float htf = H[0 * 3 + i] * FiT[0 * 3 + j] + H[1 * 3 + i] * FiT[1 * 3 + j] + H[2 * 3 + i] * FiT[2 * 3 + j];
-        P[i * 3 + j] = mu * (H[i * 3 + j] + htf) + lam * lnJ * FiT[i * 3 + j];
-      }
+    for (int j = 0; j < ND; ++j) P[j] = H[j];
   }
-  if (A) {
-    const float c1 = mu - lam * lnJ;
+  template <int ND>
+  __device__ static void prep(const MatParams&, const float*, const float* g, float* v) {
 #pragma unroll
-    for (int r = 0; r < 9; ++r)
-#pragma unroll
-      for (int c = r; c < 9; ++c) {
-        const int i = r / 3, J_ = r % 3, k = c / 3, L = c % 3;
-        float v = c1 * FiT[i * 3 + L] * FiT[k * 3 + J_] + lam * FiT[i * 3 + J_] * FiT[k * 3 + L];
-        if (i == k && J_ == L) v += mu;
-        A[asym_idx(r, c)] = v;
-      }
+    for (int j = 0; j < ND; ++j) v[j] = g[j];
   }
-}
+  template <int ND>
+  __device__ static void pair(const MatParams&, const float*, const float* va, const float* vb, float w, float* K) {
+    float s = 0;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) s += va[j] * vb[j];
+    K[0] += w * s;
+  }
+  template <int ND>
+  __device__ static void dP(const MatParams&, const float*, const float* Gv, float* out) {
+#pragma unroll
+    for (int j = 0; j < ND; ++j) out[j] = Gv[j];
+  }
+};
 
-// Small-strain linear elasticity of the reference's test Mechanics physics: psi = K/2 tr(eps)^2 + G dev eps : dev eps
-// (test/mechanics/TestMechanicsCommon.jl:14-20).
-__device__ __forceinline__ void linear_elastic(float K, float G, const float* H, float* P, float* A) {
-  const float tr = H[0] + H[4] + H[8];
-  const float lm = K - 2.0f * G / 3.0f;
-  if (P) {
+// Small-strain linear elasticity of the reference's test Mechanics physics, psi = K/2 tr(eps)^2 + G dev eps : dev eps
+// (test/mechanics/TestMechanicsCommon.jl:14-20): A = (K - 2G/3) I(x)I + G (d_ik d_jl + d_il d_jk).
+template <>
+struct Mat<FECX_LINEAR_ELASTIC> {
+  static constexpr int NF = 3, NC = 1, NV = 3;
+  template <int ND>
+  __device__ static void eval(const MatParams& mp, const float* H, const float*, float*, float* P, float*, bool&) {
+    const float K = mp.p[1], G = mp.p[2], lm = K - 2.0f * G / 3.0f;
+    const float tr = H[0] + H[4] + H[8];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) P[i * 3 + j] = G * (H[i * 3 + j] + H[j * 3 + i]) + (i == j ? lm * tr : 0.0f);
   }
-  if (A) {
-#pragma unroll
-    for (int r = 0; r < 9; ++r)
-#pragma unroll
-      for (int c = r; c < 9; ++c) {
-        const int i = r / 3, j = r % 3, k = c / 3, l = c % 3;
-        A[asym_idx(r, c)] = lm * (i == j) * (k == l) + G * ((i == k) * (j == l) + (i == l) * (j == k));
-      }
+  template <int ND>
+  __device__ static void prep(const MatParams&, const float*, const float* g, float* v) {
+    v[0] = g[0]; v[1] = g[1]; v[2] = g[2];
   }
-}
+  template <int ND>
+  __device__ static void pair(const MatParams& mp, const float*, const float* ga, const float* gb, float w, float* Kb) {
+    const float G = mp.p[2], lm = mp.p[1] - 2.0f * G / 3.0f;
+    const float gg = w * G * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) Kb[i * 3 + k] += w * (lm * ga[i] * gb[k] + G * ga[k] * gb[i]) + (i == k ? gg : 0.0f);
+  }
+  template <int ND>
+  __device__ static void dP(const MatParams& mp, const float*, const float* Gv, float* out) {
+    const float G = mp.p[2], lm = mp.p[1] - 2.0f * G / 3.0f;
+    const float tr = Gv[0] + Gv[4] + Gv[8];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) out[i * 3 + j] = G * (Gv[i * 3 + j] + Gv[j * 3 + i]) + (i == j ? lm * tr : 0.0f);
+  }
+};
 
-// J2 small-strain plasticity with linear isotropic hardening, one-step radial return and its consistent
-// tangent (Simo & Hughes, Computational Inelasticity, Box 3.1-3.2). State (7): plastic strain in
-// SymmetricTensor order (11,21,31,22,32,33), then equivalent plastic strain.
-__device__ __forceinline__ void j2(const float* p, const float* H, const float* so, float* sn, float* P, float* A) {
+// Compressible neo-Hookean, psi = mu/2 (I1 - 3) - mu ln J + lambda/2 (ln J)^2, written in the displacement gradient H
+// so that no term cancels for small H: J - 1 = tr H + (tr^2 H - tr H^2)/2 + det H, ln J = log1p(J - 1),
+// P = mu H + mu H^T F^-T + lambda ln J F^-T.
+// A_iJkL = mu d_ik d_JL + (mu - lambda ln J) Finv_Li Finv_Jk + lambda Finv_Ji Finv_Lk, so with h = F^-1 g
+// (h_i = sum_J F^-T_iJ g_J): K_ab = mu (g_a . g_b) I + (mu - lambda ln J) h_b (x) h_a + lambda h_a (x) h_b.
+// C = {F^-T (9, row-major), mu - lambda ln J}.
+template <>
+struct Mat<FECX_NEOHOOKEAN> {
+  static constexpr int NF = 3, NC = 10, NV = 6;
+  template <int ND>
+  __device__ static void eval(const MatParams& mp, const float* H, const float*, float*, float* P, float* C, bool& bad) {
+    const float lam = mp.p[0], mu = mp.p[1];
+    const float tr = H[0] + H[4] + H[8];
+    const float trH2 = H[0] * H[0] + H[4] * H[4] + H[8] * H[8] + 2.0f * (H[1] * H[3] + H[2] * H[6] + H[5] * H[7]);
+    const float detH = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
+                       H[2] * (H[3] * H[7] - H[4] * H[6]);
+    const float Jm1 = tr + 0.5f * (tr * tr - trH2) + detH;
+    const float J = 1.0f + Jm1;
+    if (!(J > 0.0f)) bad = true;
+    const float lnJ = log1pf(Jm1);
+    const float F[9] = {1 + H[0], H[1], H[2], H[3], 1 + H[4], H[5], H[6], H[7], 1 + H[8]};
+    const float iJ = 1.0f / J;
+    float FiT[9];  // cofactor(F) / J
+    FiT[0] = (F[4] * F[8] - F[5] * F[7]) * iJ; FiT[1] = (F[5] * F[6] - F[3] * F[8]) * iJ;
+    FiT[2] = (F[3] * F[7] - F[4] * F[6]) * iJ; FiT[3] = (F[2] * F[7] - F[1] * F[8]) * iJ;
+    FiT[4] = (F[0] * F[8] - F[2] * F[6]) * iJ; FiT[5] = (F[1] * F[6] - F[0] * F[7]) * iJ;
+    FiT[6] = (F[1] * F[5] - F[2] * F[4]) * iJ; FiT[7] = (F[2] * F[3] - F[0] * F[5]) * iJ;
+    FiT[8] = (F[0] * F[4] - F[1] * F[3]) * iJ;
+    if (P) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const float htf = H[0 * 3 + i] * FiT[0 * 3 + j] + H[1 * 3 + i] * FiT[1 * 3 + j] + H[2 * 3 + i] * FiT[2 * 3 + j];
+          P[i * 3 + j] = mu * (H[i * 3 + j] + htf) + lam * lnJ * FiT[i * 3 + j];
+        }
+    }
+    if (C) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) C[i] = FiT[i];
+      C[9] = mu - lam * lnJ;
+    }
+  }
+  template <int ND>
+  __device__ static void prep(const MatParams&, const float* C, const float* g, float* v) {
+    v[0] = g[0]; v[1] = g[1]; v[2] = g[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[3 + i] = C[i * 3 + 0] * g[0] + C[i * 3 + 1] * g[1] + C[i * 3 + 2] * g[2];
+  }
+  template <int ND>
+  __device__ static void pair(const MatParams& mp, const float* C, const float* va, const float* vb, float w, float* Kb) {
+    const float lam = mp.p[0], mu = mp.p[1], c1 = C[9];
+    const float gg = w * mu * (va[0] * vb[0] + va[1] * vb[1] + va[2] * vb[2]);
+    const float* ha = va + 3;
+    const float* hb = vb + 3;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) Kb[i * 3 + k] += w * (c1 * hb[i] * ha[k] + lam * ha[i] * hb[k]) + (i == k ? gg : 0.0f);
+  }
+  template <int ND>
+  __device__ static void dP(const MatParams& mp, const float* C, const float* Gv, float* out) {
+    // mu Gv + c1 F^-T Gv^T F^-T + lambda (F^-T : Gv) F^-T
+    const float lam = mp.p[0], mu = mp.p[1], c1 = C[9];
+    float M[9];  // Gv^T F^-T
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) M[l * 3 + j] = Gv[0 * 3 + l] * C[0 * 3 + j] + Gv[1 * 3 + l] * C[1 * 3 + j] + Gv[2 * 3 + l] * C[2 * 3 + j];
+    float tr = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) tr += C[i] * Gv[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        out[i * 3 + j] = mu * Gv[i * 3 + j] + c1 * (C[i * 3 + 0] * M[0 * 3 + j] + C[i * 3 + 1] * M[1 * 3 + j] + C[i * 3 + 2] * M[2 * 3 + j]) +
+                         lam * tr * C[i * 3 + j];
+  }
+};
+
+// J2 small-strain plasticity with linear isotropic hardening, one-step radial return and its consistent tangent
+// (Simo & Hughes, Computational Inelasticity, Box 3.1-3.2). State (7): plastic strain in SymmetricTensor order
+// (11,21,31,22,32,33), then equivalent plastic strain. Tangent C = kappa I(x)I + 2 mu theta (I_sym - I(x)I/3)
+// - 2 mu thetabar n(x)n, so with m = n g: K_ab = b1 g_a (x) g_b + b2 ((g_a . g_b) I + g_b (x) g_a) - b3 m_a (x) m_b,
+// b1 = kappa - 2 mu theta / 3, b2 = mu theta, b3 = 2 mu thetabar. C = {n (11,22,33,12,23,13), b1, b2, b3}.
+__device__ __forceinline__ void j2_update(const float* p, const float* H, const float* so, float* sn, float* P, float* C) {
   const float E = p[0], nu = p[1], sy = p[2], Hh = p[3];
   const float mu = E / (2 * (1 + nu)), kap = E / (3 * (1 - 2 * nu));
   const float ep[9] = {so[0], so[1], so[2], so[1], so[3], so[4], so[2], so[4], so[5]};
@@ -190,11 +282,12 @@ __device__ __forceinline__ void j2(const float* p, const float* H, const float* 
   const float snorm = sqrtf(sn2);
   const float r23 = 0.81649658092772603f;
   const float f = snorm - r23 * (sy + Hh * so[6]);
-  float dg = 0.0f, nh[9];
-  const float inv = f > 0.0f ? 1.0f / snorm : 0.0f;
+  const bool plastic = f > 0.0f;
+  const float inv = plastic ? 1.0f / snorm : 0.0f;
+  float nh[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) nh[i] = s[i] * inv;
-  if (f > 0.0f) dg = f / (2 * mu + 2.0f * Hh / 3.0f);
+  const float dg = plastic ? f / (2 * mu + 2.0f * Hh / 3.0f) : 0.0f;
   if (P) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) P[i] = s[i] - 2 * mu * dg * nh[i];
@@ -205,57 +298,96 @@ __device__ __forceinline__ void j2(const float* p, const float* H, const float* 
     sn[3] = ep[4] + dg * nh[4]; sn[4] = ep[7] + dg * nh[7]; sn[5] = ep[8] + dg * nh[8];
     sn[6] = so[6] + r23 * dg;
   }
-  if (A) {
-    float th = 1.0f, thb = 0.0f;
-    if (f > 0.0f) {
-      th = 1.0f - 2 * mu * dg / snorm;
-      thb = 1.0f / (1.0f + Hh / (3 * mu)) - (1.0f - th);
-    }
-#pragma unroll
-    for (int r = 0; r < 9; ++r)
-#pragma unroll
-      for (int c = r; c < 9; ++c) {
-        const int i = r / 3, j = r % 3, k = c / 3, l = c % 3;
-        const float dij = i == j, dkl = k == l;
-        A[asym_idx(r, c)] = kap * dij * dkl +
-                            2 * mu * th * (0.5f * ((i == k) * (j == l) + (i == l) * (j == k)) - dij * dkl / 3.0f) -
-                            2 * mu * thb * nh[r] * nh[c];
-      }
+  if (C) {
+    const float th = plastic ? 1.0f - 2 * mu * dg / snorm : 1.0f;
+    const float thb = plastic ? 1.0f / (1.0f + Hh / (3 * mu)) - (1.0f - th) : 0.0f;
+    C[0] = nh[0]; C[1] = nh[4]; C[2] = nh[8]; C[3] = nh[1]; C[4] = nh[5]; C[5] = nh[2];
+    C[6] = kap - 2 * mu * th / 3; C[7] = mu * th; C[8] = 2 * mu * thb;
   }
 }
 
+template <>
+struct Mat<FECX_J2> {
+  static constexpr int NF = 3, NC = 9, NV = 6;
+  template <int ND>
+  __device__ static void eval(const MatParams& mp, const float* H, const float* so, float* sn, float* P, float* C, bool&) {
+    j2_update(mp.p, H, so, sn, P, C);
+  }
+  template <int ND>
+  __device__ static void prep(const MatParams&, const float* C, const float* g, float* v) {
+    v[0] = g[0]; v[1] = g[1]; v[2] = g[2];
+    v[3] = C[0] * g[0] + C[3] * g[1] + C[5] * g[2];
+    v[4] = C[3] * g[0] + C[1] * g[1] + C[4] * g[2];
+    v[5] = C[5] * g[0] + C[4] * g[1] + C[2] * g[2];
+  }
+  template <int ND>
+  __device__ static void pair(const MatParams&, const float* C, const float* va, const float* vb, float w, float* Kb) {
+    const float b1 = C[6], b2 = C[7], b3 = C[8];
+    const float gg = w * b2 * (va[0] * vb[0] + va[1] * vb[1] + va[2] * vb[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        Kb[i * 3 + k] += w * (b1 * va[i] * vb[k] + b2 * va[k] * vb[i] - b3 * va[3 + i] * vb[3 + k]) + (i == k ? gg : 0.0f);
+  }
+  template <int ND>
+  __device__ static void dP(const MatParams&, const float* C, const float* Gv, float* out) {
+    const float b1 = C[6], b2 = C[7], b3 = C[8];
+    const float tr = Gv[0] + Gv[4] + Gv[8];
+    const float n[9] = {C[0], C[3], C[5], C[3], C[1], C[4], C[5], C[4], C[2]};
+    float nd = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) nd += n[i] * Gv[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        out[i * 3 + j] = b2 * (Gv[i * 3 + j] + Gv[j * 3 + i]) + (i == j ? b1 * tr : 0.0f) - b3 * nd * n[i * 3 + j];
+  }
+};
+
 // ------------------------------------------------------------------ the kernel
-template <int KIND, int NF, int PHYS>
+template <int KIND, int NF, int PHYS, int MODE>
 struct ElemLDS {
   using T = ET<KIND>;
+  static constexpr bool DO_R = (MODE & MODE_R) != 0, DO_K = (MODE & MODE_K) != 0;
+  static constexpr bool DO_MF = (MODE & MODE_ACTION_MF) != 0, DO_ASM = (MODE & MODE_ACTION_ASM) != 0;
+  static constexpr bool NEED_C = DO_K || DO_MF || DO_ASM;
   static constexpr int D = NF * T::ND;
-  static constexpr int AS = (PHYS == FECX_POISSON) ? 1 : 45;
   float x[T::NNPE][T::ND];
   float x0[T::ND];
   float u[T::NNPE][NF];
-  float v[T::NNPE][NF];
-  float kv[T::NNPE][NF];
+  float v[(DO_MF || DO_ASM) ? T::NNPE : 1][NF];
+  float kv[DO_ASM ? T::NNPE : 1][NF];
   struct QP {
     float g[T::NNPE][T::ND];
     float w;
-    float P[D];
-    float src[NF];
-    float A[AS];
-  } qp[T::NQ];
+    float P[(DO_R || DO_MF) ? D : 1];
+    float src[(DO_R && PHYS == FECX_POISSON) ? 1 : 1];
+    float C[NEED_C ? Mat<PHYS>::NC : 1];
+  };
+  // the element matrix blocks are staged over the quadrature data once that has been consumed
+  union {
+    QP qp[T::NQ];
+    float K[DO_K ? T::NPAIRS * NF * NF : 1];
+  };
 };
 
+constexpr int ELEM_THREADS = 256;  // four wavefronts, 256 / G elements per work-group
+
 template <int KIND, int NF, int PHYS, int MODE>
-__global__ __launch_bounds__(64) void element_kernel(ElementArgs args) {
+__global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   using T = ET<KIND>;
-  constexpr int ND = T::ND, NNPE = T::NNPE, NQ = T::NQ, G = T::G, EPW = 64 / G, NDOF = NNPE * NF;
+  using M = Mat<PHYS>;
+  using LDS = ElemLDS<KIND, NF, PHYS, MODE>;
+  constexpr int ND = T::ND, NNPE = T::NNPE, NQ = T::NQ, G = T::G, EPW = ELEM_THREADS / G, NDOF = NNPE * NF;
   constexpr int D = NF * ND;
-  constexpr bool DO_R = (MODE & MODE_R) != 0, DO_K = (MODE & MODE_K) != 0;
-  constexpr bool DO_MF = (MODE & MODE_ACTION_MF) != 0, DO_ASM = (MODE & MODE_ACTION_ASM) != 0;
-  constexpr bool NEED_A = DO_K || DO_MF || DO_ASM;
-  __shared__ ElemLDS<KIND, NF, PHYS> lds[EPW];
+  constexpr bool DO_R = LDS::DO_R, DO_K = LDS::DO_K, DO_MF = LDS::DO_MF, DO_ASM = LDS::DO_ASM;
+  constexpr bool NEED_C = LDS::NEED_C;
+  __shared__ LDS lds[EPW];
 
   const int lane = threadIdx.x % G, grp = threadIdx.x / G;
-  const int64_t e = (int64_t)blockIdx.x * EPW + grp;
+  const int64_t e = (int64_t)blockIdx.x * EPW + grp;  // element storage position
   const bool valid = e < args.ne;
   auto& L = lds[grp];
   bool bad = false;
@@ -274,7 +406,7 @@ __global__ __launch_bounds__(64) void element_kernel(ElementArgs args) {
         if (DO_MF || DO_ASM) vv[i] = args.V[node * NF + i];
       }
     }
-    const int src0 = grp * G;  // lane of local node 0
+    const int src0 = (grp * G) % 64;  // lane (within the wavefront) of local node 0
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       const float h0 = __shfl(hi[d], src0), l0 = __shfl(lo[d], src0);
@@ -285,11 +417,13 @@ __global__ __launch_bounds__(64) void element_kernel(ElementArgs args) {
     for (int i = 0; i < NF; ++i) {
       const double u0 = __shfl(uu[i], src0);
       if (lane < NNPE) L.u[lane][i] = (float)(uu[i] - u0);
-      if (DO_MF) {
+      if constexpr (DO_MF) {
         const double v0 = __shfl(vv[i], src0);
         if (lane < NNPE) L.v[lane][i] = (float)(vv[i] - v0);
       }
-      if (DO_ASM && lane < NNPE) { L.v[lane][i] = (float)vv[i]; L.kv[lane][i] = 0.0f; }
+      if constexpr (DO_ASM) {
+        if (lane < NNPE) { L.v[lane][i] = (float)vv[i]; L.kv[lane][i] = 0.0f; }
+      }
     }
   }
   __syncthreads();
@@ -297,8 +431,11 @@ __global__ __launch_bounds__(64) void element_kernel(ElementArgs args) {
   // ---------------- quadrature points (lane = q)
   if (valid && lane < NQ) {
     const int q = lane;
+    float xi[ND];
+    qpoint<KIND>(q, xi);
     float N[NNPE], dN[NNPE][ND];
-    shape_at<KIND>(q, N, dN);
+#pragma unroll
+    for (int a = 0; a < NNPE; ++a) shape_node<KIND>(a, xi, N[a], dN[a]);
     float J[ND][ND] = {};
 #pragma unroll
     for (int a = 0; a < NNPE; ++a)
@@ -336,7 +473,8 @@ __global__ __launch_bounds__(64) void element_kernel(ElementArgs args) {
         g[a][j] = s;
         Q.g[a][j] = s;
       }
-    Q.w = det * qweight<KIND>(q);
+    const float w = det * qweight<KIND>();
+    Q.w = w;
     float H[D] = {};
 #pragma unroll
     for (int a = 0; a < NNPE; ++a)
@@ -344,58 +482,47 @@ __global__ __launch_bounds__(64) void element_kernel(ElementArgs args) {
       for (int i = 0; i < NF; ++i)
 #pragma unroll
         for (int j = 0; j < ND; ++j) H[i * ND + j] += L.u[a][i] * g[a][j];
-    float P[D], A[(PHYS == FECX_POISSON) ? 1 : 45];
-    if constexpr (PHYS == FECX_POISSON) {
+    float P[D], C[M::NC];
+    float so[7] = {}, sn[7];
+    if constexpr (PHYS == FECX_J2) {
+      const int64_t sidx = ((int64_t)args.eperm[e] * NQ + q) * 7;
 #pragma unroll
-      for (int j = 0; j < ND; ++j) P[j] = H[j];
-      if (DO_R) {
-        float f = args.mp.p[1];
-        if (args.mp.source_kind == 1) {
-          float Xq[ND];
+      for (int s = 0; s < 7; ++s) so[s] = args.st_old[sidx + s];
+    }
+    M::template eval<ND>(args.mp, H, so, sn, P, C, bad);
+    if constexpr (PHYS == FECX_J2) {
+      if (DO_R && args.st_new) {
+        const int64_t sidx = ((int64_t)args.eperm[e] * NQ + q) * 7;
 #pragma unroll
-          for (int d = 0; d < ND; ++d) {
-            float s = 0;
-#pragma unroll
-            for (int a = 0; a < NNPE; ++a) s += N[a] * L.x[a][d];
-            Xq[d] = L.x0[d] + s;
-          }
-          f *= (float)ND * 9.8696044010893586f;
-#pragma unroll
-          for (int d = 0; d < ND; ++d) f *= sinpif(Xq[d]);
-        }
-        Q.src[0] = f;
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < NF; ++i) Q.src[i] = 0.0f;
-      if constexpr (PHYS == FECX_NEOHOOKEAN) {
-        neohookean(args.mp.p[0], args.mp.p[1], H, P, NEED_A ? A : nullptr, bad);
-      } else if constexpr (PHYS == FECX_LINEAR_ELASTIC) {
-        linear_elastic(args.mp.p[1], args.mp.p[2], H, P, NEED_A ? A : nullptr);
-      } else if constexpr (PHYS == FECX_J2) {
-        float so[7], sn[7];
-        const int64_t sidx = (e * NQ + q) * 7;
-#pragma unroll
-        for (int s = 0; s < 7; ++s) so[s] = args.st_old[sidx + s];
-        j2(args.mp.p, H, so, sn, P, NEED_A ? A : nullptr);
-        if (DO_R && args.st_new) {
-#pragma unroll
-          for (int s = 0; s < 7; ++s) args.st_new[sidx + s] = sn[s];
-        }
-      }
-      if (NEED_A) {
-#pragma unroll
-        for (int i = 0; i < 45; ++i) Q.A[i] = A[i];
+        for (int s = 0; s < 7; ++s) args.st_new[sidx + s] = sn[s];
       }
     }
 #pragma unroll
     for (int c = 0; c < D; ++c)
       if (!(P[c] == P[c])) bad = true;
-    if (DO_R) {
+    if constexpr (DO_R) {
 #pragma unroll
       for (int c = 0; c < D; ++c) Q.P[c] = P[c];
+      if constexpr (PHYS == FECX_POISSON) {
+        float f = args.mp.p[1];
+        if (args.mp.source_kind == 1) {
+          f *= (float)ND * 9.8696044010893586f;
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            float s = 0;
+#pragma unroll
+            for (int a = 0; a < NNPE; ++a) s += N[a] * L.x[a][d];
+            f *= sinpif(L.x0[d] + s);
+          }
+        }
+        Q.src[0] = f;
+      }
     }
-    if (DO_MF) {
+    if constexpr (NEED_C) {
+#pragma unroll
+      for (int c = 0; c < M::NC; ++c) Q.C[c] = C[c];
+    }
+    if constexpr (DO_MF) {
       float Gv[D] = {};
 #pragma unroll
       for (int a = 0; a < NNPE; ++a)
@@ -403,114 +530,81 @@ __global__ __launch_bounds__(64) void element_kernel(ElementArgs args) {
         for (int i = 0; i < NF; ++i)
 #pragma unroll
           for (int j = 0; j < ND; ++j) Gv[i * ND + j] += L.v[a][i] * g[a][j];
+      float out[D];
+      M::template dP<ND>(args.mp, C, Gv, out);
 #pragma unroll
-      for (int r = 0; r < D; ++r) {
-        float s = 0;
-        if constexpr (PHYS == FECX_POISSON) s = Gv[r];
-        else {
-#pragma unroll
-          for (int c = 0; c < D; ++c) s += A[asym_idx(r, c)] * Gv[c];
-        }
-        Q.P[r] = s;
-      }
+      for (int c = 0; c < D; ++c) Q.P[c] = out[c];
     }
   }
   __syncthreads();
 
   // ---------------- node rows (lane = a)
+  constexpr int MAXP = T::MAXP;
+  float kreg[DO_K ? MAXP : 1][NF * NF];
   if (valid && lane < NNPE) {
     const int a = lane;
-    if (DO_R || DO_MF) {
+    if constexpr (DO_R || DO_MF) {
       float out[NF] = {};
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const auto& Q = L.qp[q];
-        float Nq[NNPE], dNq[NNPE][ND];
-        if (DO_R && PHYS == FECX_POISSON) shape_at<KIND>(q, Nq, dNq);
+        float Na = 0.0f, dNa[ND];
+        if constexpr (DO_R && PHYS == FECX_POISSON) {
+          float xi[ND];
+          qpoint<KIND>(q, xi);
+          shape_node<KIND>(a, xi, Na, dNa);
+        }
 #pragma unroll
         for (int i = 0; i < NF; ++i) {
           float s = 0;
 #pragma unroll
           for (int j = 0; j < ND; ++j) s += Q.P[i * ND + j] * Q.g[a][j];
-          if (DO_R && PHYS == FECX_POISSON) s -= Nq[a] * Q.src[i];
+          if constexpr (DO_R && PHYS == FECX_POISSON) s -= Na * Q.src[i];
           out[i] += Q.w * s;
         }
       }
 #pragma unroll
       for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + a * NF + i] = out[i];
     }
-    if (DO_K || DO_ASM) {
-      constexpr int MAXP = T::MAXP;
+    if constexpr (DO_K || DO_ASM) {
       const int cnt = lane_cnt(NNPE, a);
       float acc[MAXP][NF * NF];
 #pragma unroll
       for (int p = 0; p < MAXP; ++p)
 #pragma unroll
         for (int c = 0; c < NF * NF; ++c) acc[p][c] = 0.0f;
-#pragma unroll 1
+#pragma unroll 2
       for (int q = 0; q < NQ; ++q) {
         const auto& Q = L.qp[q];
-        float ga[ND];
+        float C[M::NC];
 #pragma unroll
-        for (int j = 0; j < ND; ++j) ga[j] = Q.g[a][j];
-        if constexpr (PHYS == FECX_POISSON) {
-#pragma unroll
-          for (int p = 0; p < MAXP; ++p) {
-            if (p < cnt) {
-              const int b = (a + p) % NNPE;
-              float s = 0;
-#pragma unroll
-              for (int j = 0; j < ND; ++j) s += ga[j] * Q.g[b][j];
-              acc[p][0] += Q.w * s;
-            }
-          }
-        } else {
-          // T[i][k][l] = sum_j g_a[j] A[(i,j),(k,l)]
-          float Tm[NF][NF][ND];
-#pragma unroll
-          for (int i = 0; i < NF; ++i)
-#pragma unroll
-            for (int k = 0; k < NF; ++k)
-#pragma unroll
-              for (int l = 0; l < ND; ++l) {
-                float s = 0;
-#pragma unroll
-                for (int j = 0; j < ND; ++j) s += ga[j] * Q.A[asym_idx(i * ND + j, k * ND + l)];
-                Tm[i][k][l] = Q.w * s;
-              }
-#pragma unroll
-          for (int p = 0; p < MAXP; ++p) {
-            if (p < cnt) {
-              const int b = (a + p) % NNPE;
-              float gb[ND];
-#pragma unroll
-              for (int l = 0; l < ND; ++l) gb[l] = Q.g[b][l];
-#pragma unroll
-              for (int i = 0; i < NF; ++i)
-#pragma unroll
-                for (int k = 0; k < NF; ++k) {
-                  float s = 0;
-#pragma unroll
-                  for (int l = 0; l < ND; ++l) s += Tm[i][k][l] * gb[l];
-                  acc[p][i * NF + k] += s;
-                }
-            }
-          }
-        }
-      }
-      if (DO_K) {
-        float* dst = args.Kel + (e * T::NPAIRS + lane_off(NNPE, a)) * (NF * NF);
-#pragma unroll
-        for (int p = 0; p < MAXP; ++p)
-          if (p < cnt)
-#pragma unroll
-            for (int c = 0; c < NF * NF; ++c) dst[p * NF * NF + c] = acc[p][c];
-      }
-      if (DO_ASM) {
+        for (int c = 0; c < M::NC; ++c) C[c] = Q.C[c];
+        const float w = Q.w;
+        float va[M::NV];
+        M::template prep<ND>(args.mp, C, Q.g[a], va);
 #pragma unroll
         for (int p = 0; p < MAXP; ++p) {
           if (p < cnt) {
-            const int b = (a + p) % NNPE;
+            const int b = a + p < NNPE ? a + p : a + p - NNPE;
+            float vb[M::NV];
+            M::template prep<ND>(args.mp, C, Q.g[b], vb);
+            M::template pair<ND>(args.mp, C, va, vb, w, acc[p]);
+          }
+        }
+      }
+      if constexpr (DO_K) {
+        // every lane of the work-group has finished reading the quadrature data of its element only after the
+        // barrier below, so the blocks are kept in registers until then
+#pragma unroll
+        for (int p = 0; p < MAXP; ++p)
+#pragma unroll
+          for (int c = 0; c < NF * NF; ++c) kreg[p][c] = acc[p][c];
+      }
+      if constexpr (DO_ASM) {
+#pragma unroll
+        for (int p = 0; p < MAXP; ++p) {
+          if (p < cnt) {
+            const int b = a + p < NNPE ? a + p : a + p - NNPE;
 #pragma unroll
             for (int i = 0; i < NF; ++i) {
               float s = 0;
@@ -532,11 +626,36 @@ __global__ __launch_bounds__(64) void element_kernel(ElementArgs args) {
       }
     }
   }
-  if (DO_ASM) {
+  if constexpr (DO_ASM) {
     __syncthreads();
     if (valid && lane < NNPE) {
 #pragma unroll
       for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + lane * NF + i] = L.kv[lane][i];
+    }
+  }
+  if constexpr (DO_K) {
+    // stage the blocks in LDS (over the consumed quadrature data), then the whole wavefront copies its elements'
+    // contiguous storage range with consecutive lanes on consecutive words
+    __syncthreads();
+    if (valid && lane < NNPE) {
+      const int cnt = lane_cnt(NNPE, lane);
+      float* dst = L.K + pair_lane_off(NNPE, lane) * (NF * NF);
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p)
+        if (p < cnt)
+#pragma unroll
+          for (int c = 0; c < NF * NF; ++c) dst[p * NF * NF + c] = kreg[p][c];
+    }
+    __syncthreads();
+    constexpr int EW = T::NPAIRS * NF * NF;   // words per element
+    constexpr int EPWAVE = 64 / G;            // elements per wavefront
+    const int wl = threadIdx.x % 64, wv = threadIdx.x / 64;
+    const int64_t e0 = (int64_t)blockIdx.x * EPW + wv * EPWAVE;
+    const int64_t nvalid = args.ne - e0 < EPWAVE ? args.ne - e0 : EPWAVE;
+    if (nvalid > 0) {
+      float* out = args.Kel + e0 * EW;
+      const int total = (int)nvalid * EW;
+      for (int idx = wl; idx < total; idx += 64) out[idx] = lds[wv * EPWAVE + idx / EW].K[idx % EW];
     }
   }
   if (bad) atomicOr(args.flag, 1);
@@ -545,14 +664,15 @@ __global__ __launch_bounds__(64) void element_kernel(ElementArgs args) {
 template <int KIND, int NF, int PHYS>
 static void launch_modes(int mode, const ElementArgs& a, hipStream_t s) {
   using T = ET<KIND>;
-  constexpr int EPW = 64 / T::G;
+  constexpr int EPW = ELEM_THREADS / T::G;
   const unsigned grid = (unsigned)((a.ne + EPW - 1) / EPW);
+  constexpr int B = ELEM_THREADS;
   switch (mode) {
-    case MODE_R: element_kernel<KIND, NF, PHYS, MODE_R><<<grid, 64, 0, s>>>(a); break;
-    case MODE_K: element_kernel<KIND, NF, PHYS, MODE_K><<<grid, 64, 0, s>>>(a); break;
-    case MODE_R | MODE_K: element_kernel<KIND, NF, PHYS, MODE_R | MODE_K><<<grid, 64, 0, s>>>(a); break;
-    case MODE_ACTION_MF: element_kernel<KIND, NF, PHYS, MODE_ACTION_MF><<<grid, 64, 0, s>>>(a); break;
-    case MODE_ACTION_ASM: element_kernel<KIND, NF, PHYS, MODE_ACTION_ASM><<<grid, 64, 0, s>>>(a); break;
+    case MODE_R: element_kernel<KIND, NF, PHYS, MODE_R><<<grid, B, 0, s>>>(a); break;
+    case MODE_K: element_kernel<KIND, NF, PHYS, MODE_K><<<grid, B, 0, s>>>(a); break;
+    case MODE_R | MODE_K: element_kernel<KIND, NF, PHYS, MODE_R | MODE_K><<<grid, B, 0, s>>>(a); break;
+    case MODE_ACTION_MF: element_kernel<KIND, NF, PHYS, MODE_ACTION_MF><<<grid, B, 0, s>>>(a); break;
+    case MODE_ACTION_ASM: element_kernel<KIND, NF, PHYS, MODE_ACTION_ASM><<<grid, B, 0, s>>>(a); break;
     default: break;
   }
 }

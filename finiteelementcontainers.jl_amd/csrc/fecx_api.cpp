@@ -117,6 +117,7 @@ fecx::ElementArgs element_args(fecx_ctx* ctx, const double* U, const double* V) 
   fecx::ElementArgs a{};
   a.ne = ctx->ne;
   a.conn = get<int32_t>(ctx, "conn");
+  a.eperm = get<int32_t>(ctx, "eperm");
   a.xhi = get<float>(ctx, "xhi");
   a.xlo = get<float>(ctx, "xlo");
   a.U = U;
@@ -132,13 +133,16 @@ fecx::ElementArgs element_args(fecx_ctx* ctx, const double* U, const double* V) 
 
 fecx::ReduceArgs reduce_args(fecx_ctx* ctx) {
   fecx::ReduceArgs r{};
-  r.nfree = ctx->hp.nfree;
   r.nn = ctx->nn;
+  r.norder = get<int32_t>(ctx, "norder");
+  r.nnpe = ctx->nnpe;
+  r.vstride = ctx->hp.vstride;
+  r.maxdeg = ctx->hp.maxdeg;
+  r.free_rank = get<int32_t>(ctx, "free_rank");
   r.bptr = get<int64_t>(ctx, "bptr");
-  r.boff = get<int64_t>(ctx, "boff");
-  r.bent = get<uint32_t>(ctx, "bent");
   r.voff = get<int32_t>(ctx, "voff");
   r.vent = get<int32_t>(ctx, "vent");
+  r.vslot = get<uint8_t>(ctx, "vslot");
   r.Kel = get<float>(ctx, "Kel");
   r.Rel = get<float>(ctx, "Rel");
   return r;
@@ -158,16 +162,11 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   require(rc == 0, FECX_ERR_UNSUPPORTED, "no element kernel for this element kind / nf / physics");
   HIPCHK(hipGetLastError());
   fecx::ReduceArgs ra = reduce_args(ctx);
-  if (mode & (fecx::MODE_R | fecx::MODE_ACTION_MF | fecx::MODE_ACTION_ASM)) {
-    ra.R = R;
-    prof.run("reduce_vector", [&] { rc = fecx::launch_reduce_vector(ctx->nf, ra, ctx->stream); });
-    HIPCHK(hipGetLastError());
-  }
-  if (mode & fecx::MODE_K) {
-    ra.vals = vals;
-    prof.run("reduce_matrix", [&] { rc = fecx::launch_reduce_matrix(ctx->nf, ra, ctx->stream); });
-    HIPCHK(hipGetLastError());
-  }
+  if (mode & (fecx::MODE_R | fecx::MODE_ACTION_MF | fecx::MODE_ACTION_ASM)) ra.R = R;
+  if (mode & fecx::MODE_K) ra.vals = vals;
+  prof.run("reduce", [&] { rc = fecx::launch_reduce(ctx->nf, ra, ctx->stream); });
+  require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->stream));
   prof.finish();
   int hflag = 0;
@@ -224,13 +223,13 @@ int fecx_set_mesh(fecx_ctx* ctx, int elem_kind, int nd, int64_t nn, const double
       require(conn[i] >= 1 && conn[i] <= nn, FECX_ERR_ARG, "connectivity entry out of range");
       ctx->conn0[i] = (int32_t)(conn[i] - 1);
     }
+    ctx->coords.assign(coords, coords + nn * nd);
     // coordinates as an FP32 hi/lo pair so element-local differences keep FP64-level accuracy
     std::vector<float> hi(nn * nd), lo(nn * nd);
     for (int64_t i = 0; i < nn * nd; ++i) {
       hi[i] = (float)coords[i];
       lo[i] = (float)(coords[i] - (double)hi[i]);
     }
-    upload(ctx, "conn", ctx->conn0);
     upload(ctx, "xhi", hi);
     upload(ctx, "xlo", lo);
     ctx->has_mesh = true;
@@ -287,14 +286,22 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
     ctx->pairs = fecx::make_pairs(ctx->nnpe);
     require((ctx->ne * ctx->pairs.npairs) < (int64_t(1) << 31), FECX_ERR_ARG, "element storage exceeds 2^31 blocks");
     fecx::build_host_pattern(ctx->hp, ctx->nn, ctx->ne, ctx->nnpe, ctx->conn0.data(), ctx->nf, ctx->node_free,
-                             ctx->pairs);
+                             ctx->pairs, ctx->nd, ctx->coords.data());
     const int nf = ctx->nf;
-    ctx->nnz = ctx->hp.nnzb * nf * nf;
-    upload(ctx, "bptr", ctx->hp.bptr);
-    upload(ctx, "boff", ctx->hp.boff);
-    upload(ctx, "bent", ctx->hp.bent);
-    upload(ctx, "voff", ctx->hp.voff);
-    upload(ctx, "vent", ctx->hp.vent);
+    const auto& hp = ctx->hp;
+    require(hp.maxdeg < 255, FECX_ERR_UNSUPPORTED, "a node couples to more than 254 nodes");
+    ctx->nnz = hp.nnzb * nf * nf;
+    std::vector<int32_t> conn_s(ctx->ne * ctx->nnpe);  // connectivity in element storage order
+    for (int64_t s = 0; s < ctx->ne; ++s)
+      for (int a = 0; a < ctx->nnpe; ++a) conn_s[s * ctx->nnpe + a] = ctx->conn0[(int64_t)hp.eperm[s] * ctx->nnpe + a];
+    upload(ctx, "conn", conn_s);
+    upload(ctx, "eperm", hp.eperm);
+    upload(ctx, "norder", hp.norder);
+    upload(ctx, "free_rank", hp.free_rank);
+    upload(ctx, "bptr", hp.bptr);
+    upload(ctx, "voff", hp.voff);
+    upload(ctx, "vent", hp.vent);
+    upload(ctx, "vslot", hp.vslot);
     dev(ctx, "Rel", sizeof(float) * ctx->ne * ctx->nnpe * nf);
     dev(ctx, "Kel", sizeof(float) * ctx->ne * ctx->pairs.npairs * nf * nf);
     ctx->has_pattern = true;

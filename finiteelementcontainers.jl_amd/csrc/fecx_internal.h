@@ -32,26 +32,54 @@ struct PairTable {
 };
 PairTable make_pairs(int nnpe);
 
+// Pair (a, b) of an element -> its block in the symmetric element storage and whether that block is K_ab (true)
+// or K_ba (false). Same rule as make_pairs, usable in kernels.
+struct PairRef {
+  int slot;
+  bool ab;
+};
+__host__ __device__ constexpr int pair_lane_off(int nnpe, int a) {
+  return a * ((nnpe - 1) / 2 + 1) + ((nnpe % 2 == 0) ? (a < nnpe / 2 ? a : nnpe / 2) : 0);
+}
+__host__ __device__ inline PairRef pair_ref(int nnpe, int a, int b) {
+  const int dmax = (nnpe - 1) / 2;
+  const int d1 = (b - a + nnpe) % nnpe;
+  if (d1 <= dmax) return {pair_lane_off(nnpe, a) + d1, true};
+  const int d2 = (a - b + nnpe) % nnpe;
+  if (d2 <= dmax) return {pair_lane_off(nnpe, b) + d2, false};
+  const int lo = a < b ? a : b;  // d1 == d2 == nnpe / 2
+  return {pair_lane_off(nnpe, lo) + nnpe / 2, lo == a};
+}
+
 // ------------------------------------------------------------------ host pattern
 // Block-CSR view of the reduced CSR pattern: a "free node" has all NF dofs unknown. Free nodes are
 // ranked in node order; the unknown index of (free rank r, dof i) is NF*r + i, so every CSR row of
 // node r has NF*deg(r) columns made of deg(r) runs of NF consecutive columns.
+//
+// Processing order: nodes are ranked along a Morton (Z-order) curve of their coordinates; the reduction walks nodes
+// in that order and elements are stored sorted by the smallest Morton rank among their nodes (ties by element
+// id), so that a group of nodes reduced together reads a compact range of the element-private storage.
+// Inverse COO->CSR map, per node n: its incident (element, local node a) pairs in ascending element id
+// (the order SparseArrays.sparse! sums duplicates in), and for each incidence one byte per local node b
+// giving the column slot of node conn[e][b] in row n (0xFF: not a free node).
 struct HostPattern {
-  int nf = 0, nnpe = 0;
-  int64_t nn = 0, ne = 0, nfree = 0, nnzb = 0, ncontrib = 0;
+  int nf = 0, nnpe = 0, vstride = 0, maxdeg = 0;
+  int64_t nn = 0, ne = 0, nfree = 0, nnzb = 0;
   std::vector<int32_t> free_rank;   // nn: rank or -1
   std::vector<int32_t> free_node;   // nfree: node id (0-based)
   std::vector<int64_t> bptr;        // nfree + 1 : block row pointer
   std::vector<int32_t> bcol;        // nnzb: free rank of the column node, ascending per row
-  std::vector<int64_t> boff;        // nnzb + 1 : contributions of each block
-  std::vector<uint32_t> bent;       // ncontrib: (e * npairs + slot) | (transposed << 31)
-  std::vector<int32_t> voff;        // nn + 1 : node -> incident (element, local node)
-  std::vector<int32_t> vent;        // ne * nnpe : e * nnpe + a, ascending e
+  std::vector<int32_t> norder;      // processing position -> node (Morton order)
+  std::vector<int32_t> eperm;       // storage position -> element
+  std::vector<int32_t> epos;        // element -> storage position
+  std::vector<int32_t> voff;        // nn + 1 : node -> incidences
+  std::vector<int32_t> vent;        // storage position * nnpe + a, ascending element id per node
+  std::vector<uint8_t> vslot;       // incidences x vstride bytes: column slot per local node b
 };
 
-// conn0: 0-based node ids, ne x nnpe row-major (element-major); node_free: nn flags.
+// conn0: 0-based node ids, ne x nnpe row-major (element-major); node_free: nn flags; coords: nd x nn (FP64).
 void build_host_pattern(HostPattern& hp, int64_t nn, int64_t ne, int nnpe, const int32_t* conn0, int nf,
-                        const std::vector<char>& node_free, const PairTable& pt);
+                        const std::vector<char>& node_free, const PairTable& pt, int nd, const double* coords);
 
 // ------------------------------------------------------------------ kernels
 struct MatParams {
@@ -64,7 +92,8 @@ enum Mode { MODE_R = 1, MODE_K = 2, MODE_ACTION_MF = 4, MODE_ACTION_ASM = 8 };
 
 struct ElementArgs {
   int64_t ne;
-  const int32_t* conn;      // ne x nnpe (0-based)
+  const int32_t* conn;      // ne x nnpe (0-based), in element storage order
+  const int32_t* eperm;     // storage position -> element id (quadrature state index)
   const float* xhi;         // nd x nn
   const float* xlo;         // nd x nn
   const double* U;          // nf x nn
@@ -81,19 +110,20 @@ struct ElementArgs {
 int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t s);
 
 struct ReduceArgs {
-  int64_t nfree, nn;
+  int64_t nn;
+  const int32_t* norder;
+  int nnpe, vstride, maxdeg;
+  const int32_t* free_rank;
   const int64_t* bptr;
-  const int64_t* boff;
-  const uint32_t* bent;
   const int32_t* voff;
   const int32_t* vent;
+  const uint8_t* vslot;
   const float* Kel;
   const float* Rel;
-  double* vals;
-  double* R;
+  double* vals;             // CSR values, or nullptr
+  double* R;                // full NF x NN field, or nullptr
 };
-int launch_reduce_matrix(int nf, const ReduceArgs& a, hipStream_t s);
-int launch_reduce_vector(int nf, const ReduceArgs& a, hipStream_t s);
+int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s);
 
 // DOF plumbing
 void launch_scatter_unknowns(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s);
@@ -121,6 +151,7 @@ struct fecx_ctx {
   int kind = -1, nd = 0, nnpe = 0, nq = 0;
   int64_t nn = 0, ne = 0;
   std::vector<int32_t> conn0;  // ne x nnpe, 0-based
+  std::vector<double> coords;  // nd x nn
   // dofs
   bool has_dofs = false;
   int nf = 0;

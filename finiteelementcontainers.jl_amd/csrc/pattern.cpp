@@ -4,8 +4,9 @@
 // packed key (I<<32)|J into `permutation`, and on every stiffness() call reduces the COO storage
 // through SparseArrays.sparse! (src/assemblers/SparsityPatterns.jl:53-117, 160-231, 301-329).
 // Here the same reduction is precomputed in block form: the sorted unique (I, J) set is the CSR
-// pattern, and for every CSR block the list of element-storage blocks that sum into it (the runs of
-// equal keys in `permutation`, in ascending element order) is the inverse COO->CSR map.
+// pattern, and the inverse COO->CSR map is stored per CSR row-node: its incident (element, local node)
+// pairs in ascending element order (the runs of equal keys in `permutation`, summed in that order) and,
+// per incidence, the column slot each local node lands in.
 #include <algorithm>
 #include <numeric>
 #include <thread>
@@ -66,95 +67,136 @@ static void parallel_for(int64_t n, F f) {
   for (auto& x : th) x.join();
 }
 
+static uint64_t spread3(uint64_t x) {  // 21 bits -> every third bit
+  x &= 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffULL;
+  x = (x | x << 16) & 0x1f0000ff0000ffULL;
+  x = (x | x << 8) & 0x100f00f00f00f00fULL;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+  x = (x | x << 2) & 0x1249249249249249ULL;
+  return x;
+}
+
 void build_host_pattern(HostPattern& hp, int64_t nn, int64_t ne, int nnpe, const int32_t* conn0, int nf,
-                        const std::vector<char>& node_free, const PairTable& pt) {
+                        const std::vector<char>& node_free, const PairTable& pt, int nd, const double* coords) {
+  (void)pt;
   hp = HostPattern();
   hp.nf = nf; hp.nnpe = nnpe; hp.nn = nn; hp.ne = ne;
-  // node -> (element, local node), ascending element order (counting sort over elements in order)
+  hp.vstride = (nnpe + 3) / 4 * 4;
+  // Morton rank of every node (21 bits per coordinate over the bounding box), ties by node id
+  std::vector<int32_t> nrank(nn);
+  {
+    double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    for (int d = 0; d < nd; ++d) {
+      lo[d] = hi[d] = coords[d];
+      for (int64_t n = 0; n < nn; ++n) { lo[d] = std::min(lo[d], coords[n * nd + d]); hi[d] = std::max(hi[d], coords[n * nd + d]); }
+    }
+    std::vector<std::pair<uint64_t, int32_t>> keyed(nn);
+    parallel_for(nn, [&](int64_t b, int64_t e, int) {
+      for (int64_t n = b; n < e; ++n) {
+        uint64_t code = 0;
+        for (int d = 0; d < nd; ++d) {
+          const double span = hi[d] > lo[d] ? hi[d] - lo[d] : 1.0;
+          const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, (coords[n * nd + d] - lo[d]) / span * 2097151.0));
+          code |= spread3(q) << d;
+        }
+        keyed[n] = {code, (int32_t)n};
+      }
+    });
+    std::sort(keyed.begin(), keyed.end());
+    hp.norder.resize(nn);
+    for (int64_t i = 0; i < nn; ++i) { hp.norder[i] = keyed[i].second; nrank[keyed[i].second] = (int32_t)i; }
+  }
+  // element storage order: by smallest Morton rank of the element's nodes, ties by element id
+  std::vector<int32_t> key(ne);
+  parallel_for(ne, [&](int64_t b, int64_t e, int) {
+    for (int64_t el = b; el < e; ++el) {
+      int32_t m = nrank[conn0[el * nnpe]];
+      for (int a = 1; a < nnpe; ++a) m = std::min(m, nrank[conn0[el * nnpe + a]]);
+      key[el] = m;
+    }
+  });
+  {
+    std::vector<int64_t> cnt(nn + 1, 0);
+    for (int64_t el = 0; el < ne; ++el) cnt[key[el] + 1]++;
+    for (int64_t n = 0; n < nn; ++n) cnt[n + 1] += cnt[n];
+    hp.eperm.resize(ne);
+    hp.epos.resize(ne);
+    for (int64_t el = 0; el < ne; ++el) {
+      const int64_t s = cnt[key[el]]++;
+      hp.eperm[s] = (int32_t)el;
+      hp.epos[el] = (int32_t)s;
+    }
+  }
+  // node -> incidences, ascending element id (counting sort over elements in order)
   hp.voff.assign(nn + 1, 0);
   for (int64_t i = 0; i < ne * nnpe; ++i) hp.voff[conn0[i] + 1]++;
   for (int64_t n = 0; n < nn; ++n) hp.voff[n + 1] += hp.voff[n];
+  std::vector<int32_t> inc_el(ne * nnpe);
   hp.vent.resize(ne * nnpe);
   {
     std::vector<int32_t> fill(hp.voff.begin(), hp.voff.end() - 1);
-    for (int64_t e = 0; e < ne; ++e)
-      for (int a = 0; a < nnpe; ++a) hp.vent[fill[conn0[e * nnpe + a]]++] = (int32_t)(e * nnpe + a);
+    for (int64_t el = 0; el < ne; ++el)
+      for (int a = 0; a < nnpe; ++a) {
+        const int32_t k = fill[conn0[el * nnpe + a]]++;
+        inc_el[k] = (int32_t)el;
+        hp.vent[k] = (int32_t)((int64_t)hp.epos[el] * nnpe + a);
+      }
   }
   // free ranks
   hp.free_rank.assign(nn, -1);
   for (int64_t n = 0; n < nn; ++n)
     if (node_free[n]) { hp.free_rank[n] = (int32_t)hp.nfree; hp.free_node.push_back((int32_t)n); ++hp.nfree; }
   const int64_t nfree = hp.nfree;
-  // pass 1: degree (distinct free neighbour nodes) and contribution count per free row
-  std::vector<int64_t> deg(nfree), ctot(nfree);
+  // pass 1: degree of every free row (distinct free neighbour nodes)
+  std::vector<int64_t> deg(nfree);
   parallel_for(nfree, [&](int64_t b, int64_t e, int) {
     std::vector<int32_t> nb;
     for (int64_t r = b; r < e; ++r) {
-      int32_t n = hp.free_node[r];
+      const int32_t n = hp.free_node[r];
       nb.clear();
-      int64_t c = 0;
       for (int32_t k = hp.voff[n]; k < hp.voff[n + 1]; ++k) {
-        int64_t el = hp.vent[k] / nnpe;
+        const int64_t el = inc_el[k];
         for (int bb = 0; bb < nnpe; ++bb) {
-          int32_t m = conn0[el * nnpe + bb];
-          if (hp.free_rank[m] >= 0) { nb.push_back(hp.free_rank[m]); ++c; }
+          const int32_t m = conn0[el * nnpe + bb];
+          if (hp.free_rank[m] >= 0) nb.push_back(hp.free_rank[m]);
         }
       }
       std::sort(nb.begin(), nb.end());
       deg[r] = std::unique(nb.begin(), nb.end()) - nb.begin();
-      ctot[r] = c;
     }
   });
   hp.bptr.assign(nfree + 1, 0);
-  std::vector<int64_t> coff(nfree + 1, 0);
-  for (int64_t r = 0; r < nfree; ++r) { hp.bptr[r + 1] = hp.bptr[r] + deg[r]; coff[r + 1] = coff[r] + ctot[r]; }
+  for (int64_t r = 0; r < nfree; ++r) {
+    hp.bptr[r + 1] = hp.bptr[r] + deg[r];
+    hp.maxdeg = std::max<int>(hp.maxdeg, (int)deg[r]);
+  }
   hp.nnzb = hp.bptr[nfree];
-  hp.ncontrib = coff[nfree];
   hp.bcol.resize(hp.nnzb);
-  hp.boff.assign(hp.nnzb + 1, 0);
-  hp.bent.resize(hp.ncontrib);
-  // pass 2: columns, per-block contribution lists in ascending element order
+  hp.vslot.assign((size_t)ne * nnpe * hp.vstride, 0xFF);
+  // pass 2: columns and the per-incidence column slots
   parallel_for(nfree, [&](int64_t b, int64_t e, int) {
     std::vector<int32_t> nb;
-    std::vector<int64_t> cnt;
     for (int64_t r = b; r < e; ++r) {
-      int32_t n = hp.free_node[r];
+      const int32_t n = hp.free_node[r];
       nb.clear();
       for (int32_t k = hp.voff[n]; k < hp.voff[n + 1]; ++k) {
-        int64_t el = hp.vent[k] / nnpe;
+        const int64_t el = inc_el[k];
         for (int bb = 0; bb < nnpe; ++bb) {
-          int32_t m = conn0[el * nnpe + bb];
+          const int32_t m = conn0[el * nnpe + bb];
           if (hp.free_rank[m] >= 0) nb.push_back(hp.free_rank[m]);
         }
       }
       std::sort(nb.begin(), nb.end());
       nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
-      int64_t b0 = hp.bptr[r];
-      std::copy(nb.begin(), nb.end(), hp.bcol.begin() + b0);
-      cnt.assign(nb.size() + 1, 0);
+      std::copy(nb.begin(), nb.end(), hp.bcol.begin() + hp.bptr[r]);
       for (int32_t k = hp.voff[n]; k < hp.voff[n + 1]; ++k) {
-        int64_t el = hp.vent[k] / nnpe;
+        const int64_t el = inc_el[k];
+        uint8_t* row = &hp.vslot[(size_t)k * hp.vstride];
         for (int bb = 0; bb < nnpe; ++bb) {
-          int32_t m = conn0[el * nnpe + bb];
-          if (hp.free_rank[m] < 0) continue;
-          int64_t s = std::lower_bound(nb.begin(), nb.end(), hp.free_rank[m]) - nb.begin();
-          cnt[s + 1]++;
-        }
-      }
-      for (size_t s = 0; s < nb.size(); ++s) cnt[s + 1] += cnt[s];
-      for (size_t s = 0; s < nb.size(); ++s) hp.boff[b0 + s + 1] = coff[r] + cnt[s + 1];
-      if (r == 0) hp.boff[0] = 0;
-      std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-      for (int32_t k = hp.voff[n]; k < hp.voff[n + 1]; ++k) {
-        int64_t el = hp.vent[k] / nnpe;
-        int a = hp.vent[k] % nnpe;
-        for (int bb = 0; bb < nnpe; ++bb) {
-          int32_t m = conn0[el * nnpe + bb];
-          if (hp.free_rank[m] < 0) continue;
-          int64_t s = std::lower_bound(nb.begin(), nb.end(), hp.free_rank[m]) - nb.begin();
-          uint32_t pid = (uint32_t)(el * pt.npairs + pt.slot_of[a][bb]);
-          uint32_t tr = pt.stored_ab[a][bb] ? 0u : 1u;
-          hp.bent[coff[r] + pos[s]++] = pid | (tr << 31);
+          const int32_t fr = hp.free_rank[conn0[el * nnpe + bb]];
+          if (fr < 0) continue;
+          row[bb] = (uint8_t)(std::lower_bound(nb.begin(), nb.end(), fr) - nb.begin());
         }
       }
     }

@@ -1,108 +1,183 @@
 // Reduction kernels (phase 2 of assembly): FP64 sums of the FP32 element-private outputs.
 //
 // Matrix: the reference reduces its element-private COO storage with SparseArrays.sparse! over the sorted
-// permutation on every stiffness() call (src/assemblers/SparsityPatterns.jl:301-329). Here each thread owns one
-// NF x NF block of the CSR pattern and sums, in FP64 and in ascending element order (the order sparse! uses),
-// the element blocks listed for it by the inverse COO->CSR map, then writes the block's NF rows x NF columns
-// with plain coalesced stores: no atomics, no zero fill, every CSR value written exactly once.
-// Vector: each thread owns one node and sums its incident element rows (the atomic scatter of
+// permutation on every stiffness() call (src/assemblers/SparsityPatterns.jl:301-329). Here a group of LPR lanes
+// owns one node: lane s owns column slot s of the node's NF CSR rows, walks the node's incident elements in
+// ascending element order (the order sparse! sums duplicates in), adds the NF x NF blocks that land in its slot
+// in FP64, and writes them with plain stores -- no atomics, no zero fill, every CSR value written exactly once,
+// adjacent lanes writing adjacent columns.
+// Vector: the same groups sum the node's incident element rows into the residual field (the atomic scatter of
 // src/assemblers/Assemblers.jl:72-87 restated as an owner-computes gather).
 #include "fecx_internal.h"
 
 namespace fecx {
 
 constexpr int RED_THREADS = 256;
-constexpr int ROWS_PER_WG = 16;
+constexpr int CH = 8;            // incidences staged per pass
 
-// Work-groups are dealt to the 8 XCDs round-robin; remap so that each XCD walks a contiguous range of
-// block rows, which keeps the element blocks shared by neighbouring rows in that XCD's L2.
+// Work-groups are dealt to the 8 XCDs round-robin; remap so that each XCD walks a contiguous range of nodes,
+// which keeps the element blocks shared by neighbouring nodes in that XCD's L2.
 __device__ __forceinline__ int64_t xcd_chunk(int64_t wg, int64_t nwg) {
-  const int64_t per = (nwg + 7) / 8;
-  const int64_t x = wg % 8, i = wg / 8;
-  return x * per + i;
+  const int64_t per = nwg / 8;  // nwg is a multiple of 8
+  return (wg % 8) * per + wg / 8;
 }
 
-template <int NF>
-__global__ __launch_bounds__(RED_THREADS) void reduce_matrix_kernel(ReduceArgs a) {
-  __shared__ int64_t sptr[ROWS_PER_WG + 1];
-  const int64_t nwg = gridDim.x;
-  const int64_t chunk = xcd_chunk(blockIdx.x, nwg);
-  const int64_t r0 = chunk * ROWS_PER_WG;
-  if (r0 >= a.nfree) return;  // whole work-group exits together
-  const int64_t r1 = min(r0 + ROWS_PER_WG, a.nfree);
-  const int nr = (int)(r1 - r0);
-  if (threadIdx.x <= nr) sptr[threadIdx.x] = a.bptr[r0 + threadIdx.x];
-  __syncthreads();
-  const int64_t B0 = sptr[0], B1 = sptr[nr];
-  for (int64_t b = B0 + threadIdx.x; b < B1; b += RED_THREADS) {
-    int lo = 0, hi = nr - 1;  // row r with sptr[r] <= b < sptr[r+1]
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (sptr[mid] <= b) lo = mid; else hi = mid - 1;
-    }
-    const int64_t rb = sptr[lo], deg = sptr[lo + 1] - rb, s = b - rb;
-    double acc[NF * NF];
+// Position of byte == s in the packed slot words (at most one match: a node appears once per element), or -1.
+template <int VW>
+__device__ __forceinline__ int match_slot(const uint32_t* w, int s) {
+  const uint32_t rep = 0x01010101u * (uint32_t)s;
 #pragma unroll
-    for (int c = 0; c < NF * NF; ++c) acc[c] = 0.0;
-    const int64_t c0 = a.boff[b], c1 = a.boff[b + 1];
-    for (int64_t c = c0; c < c1; ++c) {
-      const uint32_t ent = a.bent[c];
-      const float* blk = a.Kel + (int64_t)(ent & 0x7fffffffu) * (NF * NF);
-      if (ent >> 31) {
+  for (int q = 0; q < VW; ++q) {
+    const uint32_t x = w[q] ^ rep;
+    const uint32_t m = (x - 0x01010101u) & ~x & 0x80808080u;  // lowest flagged byte is the first zero byte
+    if (m) return 4 * q + (__builtin_ctz(m) >> 3);
+  }
+  return -1;
+}
+
+// A work-group owns 256/LPR consecutive nodes (after the XCD remap). Per pass, a node's LPR lanes
+//   1. stage up to CH incidences (element storage position, local node a, column slots, residual row) in LDS,
+//   2. copy the NNPE blocks K_ab (b = 0..NNPE-1) of each staged incidence from the symmetric element storage into
+//      LDS; a per-work-group table gives, for (a, word j of the staged row), the word's offset inside the element's
+//      storage with the transposition of blocks stored as K_ba folded in,
+//   3. sum in FP64, lane s owning column slot s, the staged blocks that land in slot s, in incidence order.
+// The residual sum runs over the same staged rows in the same order.
+template <int NF, int NNPE, int LPR, int SW>
+__global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) {
+  constexpr int NPW = RED_THREADS / LPR;  // nodes per work-group
+  constexpr int NPAIRS = NNPE * (NNPE + 1) / 2;
+  constexpr int EW = NPAIRS * NF * NF;    // stored words per element
+  constexpr int VW = (NNPE + 3) / 4;      // vslot words per incidence
+  constexpr int BW = NNPE * NF * NF;      // staged words per incidence
+  __shared__ int16_t s_off[NNPE][BW];
+  __shared__ int32_t s_v[NPW][CH];
+  __shared__ uint32_t s_w[NPW][CH][VW];
+  __shared__ float s_r[NPW][CH][NF];
+  __shared__ float s_blk[NPW][CH][BW];
+  __shared__ int s_maxpass;
+  for (int i = threadIdx.x; i < NNPE * BW; i += RED_THREADS) {
+    const int la = i / BW, j = i % BW, b = j / (NF * NF), c = j % (NF * NF);
+    const PairRef pr = pair_ref(NNPE, la, b);
+    s_off[la][j] = (int16_t)(pr.slot * NF * NF + (pr.ab ? c : (c % NF) * NF + c / NF));
+  }
+  const int64_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
+  const int64_t pos = chunk * NPW + grp;
+  const bool valid = pos < a.nn;
+  const int64_t n = valid ? a.norder[pos] : 0;
+  int32_t c0 = 0, ninc = 0;
+  if (valid) { c0 = a.voff[n]; ninc = a.voff[n + 1] - c0; }
+  const int npass = (ninc + CH - 1) / CH;
+  if (threadIdx.x == 0) s_maxpass = 0;
+  __syncthreads();
+  if (lane == 0) atomicMax(&s_maxpass, npass);
+  __syncthreads();
+  const int maxpass = s_maxpass;
+  int32_t r = -1;
+  int64_t rb = 0;
+  int deg = 0;
+  if (valid && a.vals) {
+    r = a.free_rank[n];
+    if (r >= 0) { rb = a.bptr[r]; deg = (int)(a.bptr[r + 1] - rb); }
+  }
+  double acc[SW][NF * NF];
+#pragma unroll
+  for (int t = 0; t < SW; ++t)
+#pragma unroll
+    for (int c = 0; c < NF * NF; ++c) acc[t][c] = 0.0;
+  double racc = 0.0;
+  for (int pass = 0; pass < maxpass; ++pass) {
+    const int base = pass * CH;
+    const int cnt = min(CH, ninc - base);  // may be <= 0 for this node
+    if (lane < cnt) {
+      const int32_t c = c0 + base + lane;
+      const int32_t v = a.vent[c];
+      s_v[grp][lane] = v;
+      const uint32_t* sl = reinterpret_cast<const uint32_t*>(a.vslot) + (int64_t)c * VW;
+#pragma unroll
+      for (int q = 0; q < VW; ++q) s_w[grp][lane][q] = sl[q];
+      if (a.R) {
+#pragma unroll
+        for (int i = 0; i < NF; ++i) s_r[grp][lane][i] = a.Rel[(int64_t)v * NF + i];
+      }
+    }
+    __syncthreads();
+    if (r >= 0) {
+      for (int t = 0; t < cnt; ++t) {
+        const int32_t v = s_v[grp][t];
+        const float* src = a.Kel + (int64_t)(v / NNPE) * EW;
+        const int la = v % NNPE;
+        for (int j = lane; j < BW; j += LPR) s_blk[grp][t][j] = src[s_off[la][j]];
+      }
+    }
+    __syncthreads();
+    if (a.R && lane < NF) {
+      for (int t = 0; t < cnt; ++t) racc += (double)s_r[grp][t][lane];
+    }
+    if (r >= 0) {
+#pragma unroll
+      for (int sw = 0; sw < SW; ++sw) {
+        const int s = lane + sw * LPR;
+        if (s < deg) {
+          for (int t = 0; t < cnt; ++t) {
+            const int b = match_slot<VW>(s_w[grp][t], s);
+            if (b >= 0) {
+              const float* blk = &s_blk[grp][t][b * NF * NF];
+#pragma unroll
+              for (int c = 0; c < NF * NF; ++c) acc[sw][c] += (double)blk[c];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (valid && a.R && lane < NF) a.R[n * NF + lane] = racc;
+  if (r >= 0) {
+#pragma unroll
+    for (int sw = 0; sw < SW; ++sw) {
+      const int s = lane + sw * LPR;
+      if (s < deg) {
+        const int64_t base = (int64_t)NF * NF * rb + (int64_t)NF * s;
 #pragma unroll
         for (int i = 0; i < NF; ++i)
 #pragma unroll
-          for (int k = 0; k < NF; ++k) acc[i * NF + k] += (double)blk[k * NF + i];
-      } else {
-#pragma unroll
-        for (int c2 = 0; c2 < NF * NF; ++c2) acc[c2] += (double)blk[c2];
+          for (int k = 0; k < NF; ++k) a.vals[base + (int64_t)i * NF * deg + k] = acc[sw][i * NF + k];
       }
     }
-    const int64_t base = (int64_t)NF * NF * rb + (int64_t)NF * s;
-#pragma unroll
-    for (int i = 0; i < NF; ++i)
-#pragma unroll
-      for (int k = 0; k < NF; ++k) a.vals[base + (int64_t)i * NF * deg + k] = acc[i * NF + k];
   }
 }
 
-template <int NF>
-__global__ __launch_bounds__(RED_THREADS) void reduce_vector_kernel(ReduceArgs a) {
-  const int64_t n = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x;
-  if (n >= a.nn) return;
-  double acc[NF];
-#pragma unroll
-  for (int i = 0; i < NF; ++i) acc[i] = 0.0;
-  const int32_t c0 = a.voff[n], c1 = a.voff[n + 1];
-  for (int32_t c = c0; c < c1; ++c) {
-    const int64_t slot = (int64_t)a.vent[c] * NF;
-#pragma unroll
-    for (int i = 0; i < NF; ++i) acc[i] += (double)a.Rel[slot + i];
+template <int NF, int NNPE>
+static int launch_lpr(const ReduceArgs& a, hipStream_t s) {
+  // 16 lanes x 2 sweeps cover rows of up to 32 blocks (hex8: 27), 32 x 2 up to 64, 64 x 2 up to 128
+  // NF = 1: 16 lanes x 2 sweeps cover rows of up to 32 blocks (hex8: 27); NF = 3: one sweep of 32 lanes
+  const int lpr = (NF == 1 && a.maxdeg <= 32) ? 16 : (a.maxdeg <= 32 ? 32 : (a.maxdeg <= 64 ? 32 : 64));
+  if (a.maxdeg > 2 * lpr) return -1;
+  const int sw = a.maxdeg <= lpr ? 1 : 2;
+  const int64_t npw = RED_THREADS / lpr;
+  int64_t grid = (a.nn + npw - 1) / npw;
+  grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
+  const unsigned g = (unsigned)grid;
+  if (lpr == 16 && sw == 1) reduce_rows_kernel<NF, NNPE, 16, 1><<<g, RED_THREADS, 0, s>>>(a);
+  else if (lpr == 16) reduce_rows_kernel<NF, NNPE, 16, 2><<<g, RED_THREADS, 0, s>>>(a);
+  else if (lpr == 32 && sw == 1) reduce_rows_kernel<NF, NNPE, 32, 1><<<g, RED_THREADS, 0, s>>>(a);
+  else if (lpr == 32) reduce_rows_kernel<NF, NNPE, 32, 2><<<g, RED_THREADS, 0, s>>>(a);
+  else reduce_rows_kernel<NF, NNPE, 64, 2><<<g, RED_THREADS, 0, s>>>(a);
+  return 0;
+}
+
+int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s) {
+  if (a.nn == 0) return 0;
+  if (nf == 1) {
+    if (a.nnpe == 4) return launch_lpr<1, 4>(a, s);
+    if (a.nnpe == 8) return launch_lpr<1, 8>(a, s);
+    if (a.nnpe == 10) return launch_lpr<1, 10>(a, s);
+  } else if (nf == 3) {
+    if (a.nnpe == 8) return launch_lpr<3, 8>(a, s);
+    if (a.nnpe == 10) return launch_lpr<3, 10>(a, s);
   }
-#pragma unroll
-  for (int i = 0; i < NF; ++i) a.R[n * NF + i] = acc[i];
-}
-
-int launch_reduce_matrix(int nf, const ReduceArgs& a, hipStream_t s) {
-  const unsigned grid = (unsigned)((a.nfree + ROWS_PER_WG - 1) / ROWS_PER_WG);
-  if (grid == 0) return 0;
-  // the XCD remap covers chunks 0 .. 8*ceil(grid/8)-1; pad the grid so every chunk is visited
-  const unsigned padded = (unsigned)(((grid + 7) / 8) * 8);
-  if (nf == 1) reduce_matrix_kernel<1><<<padded, RED_THREADS, 0, s>>>(a);
-  else if (nf == 2) reduce_matrix_kernel<2><<<padded, RED_THREADS, 0, s>>>(a);
-  else if (nf == 3) reduce_matrix_kernel<3><<<padded, RED_THREADS, 0, s>>>(a);
-  else return -1;
-  return 0;
-}
-
-int launch_reduce_vector(int nf, const ReduceArgs& a, hipStream_t s) {
-  const unsigned grid = (unsigned)((a.nn + RED_THREADS - 1) / RED_THREADS);
-  if (grid == 0) return 0;
-  if (nf == 1) reduce_vector_kernel<1><<<grid, RED_THREADS, 0, s>>>(a);
-  else if (nf == 2) reduce_vector_kernel<2><<<grid, RED_THREADS, 0, s>>>(a);
-  else if (nf == 3) reduce_vector_kernel<3><<<grid, RED_THREADS, 0, s>>>(a);
-  else return -1;
-  return 0;
+  return -1;
 }
 
 // ------------------------------------------------------------------ DOF plumbing
