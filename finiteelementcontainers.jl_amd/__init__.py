@@ -12,6 +12,6 @@ from .assembler import (  # noqa: F401
     CSRMatrix, SparseMatrixAssembler, HostPattern, Parameters, create_parameters, create_unknowns, create_field,
     assemble_vector_, assemble_stiffness_, assemble_matrix_, assemble_vector_and_stiffness_,
     assemble_matrix_action_, assemble_matrix_free_action_, assemble_matrix_free_action_full_, hvp,
-    set_profiling, profile, reset_profile,
+    set_chunks, set_profiling, profile, reset_profile,
 )
 from . import rng, workloads  # noqa: F401

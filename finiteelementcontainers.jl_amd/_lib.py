@@ -53,6 +53,7 @@ _SIGS = {
     "fecx_update_field_unknowns": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_update_field_dirichlet": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_extract_field_unknowns": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_set_chunks": [ctypes.c_void_p, ctypes.c_int],
     "fecx_set_profiling": [ctypes.c_void_p, ctypes.c_int],
     "fecx_profile_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],
     "fecx_profile_entry": [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,

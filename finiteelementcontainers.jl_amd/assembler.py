@@ -418,6 +418,11 @@ def hvp(asm, v):
     return asm.hvp(v)
 
 
+def set_chunks(asm, chunks=0):
+    """Number of pipelined pieces for matrix assemblies (0 or 1: one piece); results are unchanged."""
+    asm.ctx.call("fecx_set_chunks", ctypes.c_int(chunks))
+
+
 def set_profiling(asm, on=True):
     asm.ctx.call("fecx_set_profiling", ctypes.c_int(1 if on else 0))
 

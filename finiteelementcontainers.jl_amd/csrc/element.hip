@@ -103,9 +103,10 @@ __device__ __forceinline__ void shape_node(int a, const float* xi, float& N, flo
 // Each material provides:
 //   NC                         floats of compact tangent data per quadrature point
 //   eval(mp, H, so, sn, P, C)  stress P (NF x ND) and tangent data C from the field gradient H (and state)
-//   prep(mp, C, g, v)          per-node vector v (NV floats) from the shape gradient g: K_ab depends on the
-//                              pair only through v_a, v_b
-//   pair(mp, C, va, vb, w, K)  K += w * (g_a . A . g_b) as an NF x NF block
+//   prep(mp, C, g, v)          per-node vector v (NV floats, v[0..ND) = g) from the shape gradient g: K_ab depends
+//                              on the pair only through v_a, v_b
+//   NA, side(mp, C, w, va, s)  the a-side factors s (NA floats) of one quadrature point, weight folded in
+//   pair(s, vb, K)             K += w * (g_a . A . g_b) as an NF x NF block
 //   dP(mp, C, Gv, out)         out = A : Gv (matrix-free action)
 template <int PHYS>
 struct Mat;
@@ -113,7 +114,7 @@ struct Mat;
 // -div(grad u) = f (test/poisson/TestPoissonCommon.jl:75-127): P = grad u, A = I
 template <>
 struct Mat<FECX_POISSON> {
-  static constexpr int NF = 1, NC = 1, NV = 3;
+  static constexpr int NF = 1, NC = 1, NV = 3, NS = 0, S0 = 0;  // side scalars: C[S0 .. S0+NS)
   template <int ND>
   __device__ static void eval(const MatParams&, const float* H, const float*, float*, float* P, float*, bool&) {
 #pragma unroll
@@ -124,12 +125,16 @@ struct Mat<FECX_POISSON> {
 #pragma unroll
     for (int j = 0; j < ND; ++j) v[j] = g[j];
   }
+  static constexpr int NA = 3;
   template <int ND>
-  __device__ static void pair(const MatParams&, const float*, const float* va, const float* vb, float w, float* K) {
-    float s = 0;
+  __device__ static void side(const MatParams&, const float*, float w, const float* va, float* sa) {
 #pragma unroll
-    for (int j = 0; j < ND; ++j) s += va[j] * vb[j];
-    K[0] += w * s;
+    for (int j = 0; j < ND; ++j) sa[j] = w * va[j];
+  }
+  template <int ND>
+  __device__ static void pair(const float* sa, const float* vb, float* K) {
+#pragma unroll
+    for (int j = 0; j < ND; ++j) K[0] = fmaf(sa[j], vb[j], K[0]);
   }
   template <int ND>
   __device__ static void dP(const MatParams&, const float*, const float* Gv, float* out) {
@@ -142,7 +147,7 @@ struct Mat<FECX_POISSON> {
 // (test/mechanics/TestMechanicsCommon.jl:14-20): A = (K - 2G/3) I(x)I + G (d_ik d_jl + d_il d_jk).
 template <>
 struct Mat<FECX_LINEAR_ELASTIC> {
-  static constexpr int NF = 3, NC = 1, NV = 3;
+  static constexpr int NF = 3, NC = 1, NV = 3, NS = 0, S0 = 0;
   template <int ND>
   __device__ static void eval(const MatParams& mp, const float* H, const float*, float*, float* P, float*, bool&) {
     const float K = mp.p[1], G = mp.p[2], lm = K - 2.0f * G / 3.0f;
@@ -156,14 +161,24 @@ struct Mat<FECX_LINEAR_ELASTIC> {
   __device__ static void prep(const MatParams&, const float*, const float* g, float* v) {
     v[0] = g[0]; v[1] = g[1]; v[2] = g[2];
   }
+  static constexpr int NA = 6;  // {w lm g_a, w G g_a}
   template <int ND>
-  __device__ static void pair(const MatParams& mp, const float*, const float* ga, const float* gb, float w, float* Kb) {
+  __device__ static void side(const MatParams& mp, const float*, float w, const float* ga, float* sa) {
     const float G = mp.p[2], lm = mp.p[1] - 2.0f * G / 3.0f;
-    const float gg = w * G * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { sa[j] = w * lm * ga[j]; sa[3 + j] = w * G * ga[j]; }
+  }
+  template <int ND>
+  __device__ static void pair(const float* sa, const float* gb, float* Kb) {
+    const float gg = fmaf(sa[3], gb[0], fmaf(sa[4], gb[1], sa[5] * gb[2]));
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) Kb[i * 3 + k] += w * (lm * ga[i] * gb[k] + G * ga[k] * gb[i]) + (i == k ? gg : 0.0f);
+      for (int k = 0; k < 3; ++k) {
+        float t = fmaf(sa[i], gb[k], Kb[i * 3 + k]);
+        t = fmaf(gb[i], sa[3 + k], t);
+        Kb[i * 3 + k] = i == k ? t + gg : t;
+      }
   }
   template <int ND>
   __device__ static void dP(const MatParams& mp, const float*, const float* Gv, float* out) {
@@ -184,7 +199,7 @@ struct Mat<FECX_LINEAR_ELASTIC> {
 // C = {F^-T (9, row-major), mu - lambda ln J}.
 template <>
 struct Mat<FECX_NEOHOOKEAN> {
-  static constexpr int NF = 3, NC = 10, NV = 6;
+  static constexpr int NF = 3, NC = 10, NV = 6, NS = 1, S0 = 9;
   template <int ND>
   __device__ static void eval(const MatParams& mp, const float* H, const float*, float*, float* P, float* C, bool& bad) {
     const float lam = mp.p[0], mu = mp.p[1];
@@ -225,16 +240,25 @@ struct Mat<FECX_NEOHOOKEAN> {
 #pragma unroll
     for (int i = 0; i < 3; ++i) v[3 + i] = C[i * 3 + 0] * g[0] + C[i * 3 + 1] * g[1] + C[i * 3 + 2] * g[2];
   }
+  static constexpr int NA = 9;  // {w mu g_a, w c1 h_a, w lambda h_a}
   template <int ND>
-  __device__ static void pair(const MatParams& mp, const float* C, const float* va, const float* vb, float w, float* Kb) {
-    const float lam = mp.p[0], mu = mp.p[1], c1 = C[9];
-    const float gg = w * mu * (va[0] * vb[0] + va[1] * vb[1] + va[2] * vb[2]);
-    const float* ha = va + 3;
+  __device__ static void side(const MatParams& mp, const float* C, float w, const float* va, float* sa) {
+    const float wl = w * mp.p[0], wm = w * mp.p[1], wc = w * C[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { sa[j] = wm * va[j]; sa[3 + j] = wc * va[3 + j]; sa[6 + j] = wl * va[3 + j]; }
+  }
+  template <int ND>
+  __device__ static void pair(const float* sa, const float* vb, float* Kb) {
+    const float gg = fmaf(sa[0], vb[0], fmaf(sa[1], vb[1], sa[2] * vb[2]));
     const float* hb = vb + 3;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) Kb[i * 3 + k] += w * (c1 * hb[i] * ha[k] + lam * ha[i] * hb[k]) + (i == k ? gg : 0.0f);
+      for (int k = 0; k < 3; ++k) {
+        float t = fmaf(hb[i], sa[3 + k], Kb[i * 3 + k]);
+        t = fmaf(sa[6 + i], hb[k], t);
+        Kb[i * 3 + k] = i == k ? t + gg : t;
+      }
   }
   template <int ND>
   __device__ static void dP(const MatParams& mp, const float* C, const float* Gv, float* out) {
@@ -308,7 +332,7 @@ __device__ __forceinline__ void j2_update(const float* p, const float* H, const 
 
 template <>
 struct Mat<FECX_J2> {
-  static constexpr int NF = 3, NC = 9, NV = 6;
+  static constexpr int NF = 3, NC = 9, NV = 6, NS = 3, S0 = 6;
   template <int ND>
   __device__ static void eval(const MatParams& mp, const float* H, const float* so, float* sn, float* P, float* C, bool&) {
     j2_update(mp.p, H, so, sn, P, C);
@@ -320,15 +344,25 @@ struct Mat<FECX_J2> {
     v[4] = C[3] * g[0] + C[1] * g[1] + C[4] * g[2];
     v[5] = C[5] * g[0] + C[4] * g[1] + C[2] * g[2];
   }
+  static constexpr int NA = 9;  // {w b1 g_a, w b2 g_a, -w b3 m_a}
   template <int ND>
-  __device__ static void pair(const MatParams&, const float* C, const float* va, const float* vb, float w, float* Kb) {
-    const float b1 = C[6], b2 = C[7], b3 = C[8];
-    const float gg = w * b2 * (va[0] * vb[0] + va[1] * vb[1] + va[2] * vb[2]);
+  __device__ static void side(const MatParams&, const float* C, float w, const float* va, float* sa) {
+    const float w1 = w * C[6], w2 = w * C[7], w3 = -w * C[8];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { sa[j] = w1 * va[j]; sa[3 + j] = w2 * va[j]; sa[6 + j] = w3 * va[3 + j]; }
+  }
+  template <int ND>
+  __device__ static void pair(const float* sa, const float* vb, float* Kb) {
+    const float gg = fmaf(sa[3], vb[0], fmaf(sa[4], vb[1], sa[5] * vb[2]));
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        Kb[i * 3 + k] += w * (b1 * va[i] * vb[k] + b2 * va[k] * vb[i] - b3 * va[3 + i] * vb[3 + k]) + (i == k ? gg : 0.0f);
+      for (int k = 0; k < 3; ++k) {
+        float t = fmaf(sa[i], vb[k], Kb[i * 3 + k]);
+        t = fmaf(vb[i], sa[3 + k], t);
+        t = fmaf(sa[6 + i], vb[3 + k], t);
+        Kb[i * 3 + k] = i == k ? t + gg : t;
+      }
   }
   template <int ND>
   __device__ static void dP(const MatParams&, const float* C, const float* Gv, float* out) {
@@ -360,11 +394,11 @@ struct ElemLDS {
   float v[(DO_MF || DO_ASM) ? T::NNPE : 1][NF];
   float kv[DO_ASM ? T::NNPE : 1][NF];
   struct QP {
-    float g[T::NNPE][T::ND];
+    float v[T::NNPE][(DO_K || DO_ASM) ? Mat<PHYS>::NV : T::ND];  // v[a][0..ND) = shape gradient g_a
     float w;
     float P[(DO_R || DO_MF) ? D : 1];
     float src[(DO_R && PHYS == FECX_POISSON) ? 1 : 1];
-    float C[NEED_C ? Mat<PHYS>::NC : 1];
+    float S[Mat<PHYS>::NS > 0 ? Mat<PHYS>::NS : 1];  // side scalars of the tangent (C[S0 .. S0+NS))
   };
   // the element matrix blocks are staged over the quadrature data once that has been consumed
   union {
@@ -387,7 +421,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   __shared__ LDS lds[EPW];
 
   const int lane = threadIdx.x % G, grp = threadIdx.x / G;
-  const int64_t e = (int64_t)blockIdx.x * EPW + grp;  // element storage position
+  const int64_t e = args.ebeg + (int64_t)blockIdx.x * EPW + grp;  // element storage position
   const bool valid = e < args.ne;
   auto& L = lds[grp];
   bool bad = false;
@@ -471,7 +505,6 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
 #pragma unroll
         for (int k = 0; k < ND; ++k) s += dN[a][k] * Ji[k][j];
         g[a][j] = s;
-        Q.g[a][j] = s;
       }
     const float w = det * qweight<KIND>();
     Q.w = w;
@@ -518,9 +551,21 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         Q.src[0] = f;
       }
     }
-    if constexpr (NEED_C) {
+    if constexpr (DO_K || DO_ASM) {
 #pragma unroll
-      for (int c = 0; c < M::NC; ++c) Q.C[c] = C[c];
+      for (int c = 0; c < M::NS; ++c) Q.S[c] = C[M::S0 + c];
+#pragma unroll
+      for (int a = 0; a < NNPE; ++a) {
+        float va[M::NV];
+        M::template prep<ND>(args.mp, C, g[a], va);
+#pragma unroll
+        for (int c = 0; c < M::NV; ++c) Q.v[a][c] = va[c];
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < NNPE; ++a)
+#pragma unroll
+        for (int j = 0; j < ND; ++j) Q.v[a][j] = g[a][j];
     }
     if constexpr (DO_MF) {
       float Gv[D] = {};
@@ -558,7 +603,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         for (int i = 0; i < NF; ++i) {
           float s = 0;
 #pragma unroll
-          for (int j = 0; j < ND; ++j) s += Q.P[i * ND + j] * Q.g[a][j];
+          for (int j = 0; j < ND; ++j) s += Q.P[i * ND + j] * Q.v[a][j];
           if constexpr (DO_R && PHYS == FECX_POISSON) s -= Na * Q.src[i];
           out[i] += Q.w * s;
         }
@@ -578,17 +623,19 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         const auto& Q = L.qp[q];
         float C[M::NC];
 #pragma unroll
-        for (int c = 0; c < M::NC; ++c) C[c] = Q.C[c];
-        const float w = Q.w;
-        float va[M::NV];
-        M::template prep<ND>(args.mp, C, Q.g[a], va);
+        for (int c = 0; c < M::NS; ++c) C[M::S0 + c] = Q.S[c];
+        float va[M::NV], sa[M::NA];
+#pragma unroll
+        for (int c = 0; c < M::NV; ++c) va[c] = Q.v[a][c];
+        M::template side<ND>(args.mp, C, Q.w, va, sa);
 #pragma unroll
         for (int p = 0; p < MAXP; ++p) {
           if (p < cnt) {
             const int b = a + p < NNPE ? a + p : a + p - NNPE;
             float vb[M::NV];
-            M::template prep<ND>(args.mp, C, Q.g[b], vb);
-            M::template pair<ND>(args.mp, C, va, vb, w, acc[p]);
+#pragma unroll
+            for (int c = 0; c < M::NV; ++c) vb[c] = Q.v[b][c];
+            M::template pair<ND>(sa, vb, acc[p]);
           }
         }
       }
@@ -650,7 +697,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     constexpr int EW = T::NPAIRS * NF * NF;   // words per element
     constexpr int EPWAVE = 64 / G;            // elements per wavefront
     const int wl = threadIdx.x % 64, wv = threadIdx.x / 64;
-    const int64_t e0 = (int64_t)blockIdx.x * EPW + wv * EPWAVE;
+    const int64_t e0 = args.ebeg + (int64_t)blockIdx.x * EPW + wv * EPWAVE;
     const int64_t nvalid = args.ne - e0 < EPWAVE ? args.ne - e0 : EPWAVE;
     if (nvalid > 0) {
       float* out = args.Kel + e0 * EW;
@@ -665,7 +712,8 @@ template <int KIND, int NF, int PHYS>
 static void launch_modes(int mode, const ElementArgs& a, hipStream_t s) {
   using T = ET<KIND>;
   constexpr int EPW = ELEM_THREADS / T::G;
-  const unsigned grid = (unsigned)((a.ne + EPW - 1) / EPW);
+  if (a.ne <= a.ebeg) return;
+  const unsigned grid = (unsigned)((a.ne - a.ebeg + EPW - 1) / EPW);
   constexpr int B = ELEM_THREADS;
   switch (mode) {
     case MODE_R: element_kernel<KIND, NF, PHYS, MODE_R><<<grid, B, 0, s>>>(a); break;

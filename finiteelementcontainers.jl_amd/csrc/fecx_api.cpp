@@ -73,14 +73,14 @@ struct Prof {
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev;
   explicit Prof(fecx_ctx* c) : ctx(c) {}
   template <class F>
-  void run(const char* name, F f) {
+  void run(const char* name, hipStream_t st, F f) {
     if (!ctx->profiling) { f(); return; }
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
-    HIPCHK(hipEventRecord(a, ctx->stream));
+    HIPCHK(hipEventRecord(a, st));
     f();
-    HIPCHK(hipEventRecord(b, ctx->stream));
+    HIPCHK(hipEventRecord(b, st));
     ev.push_back({name, {a, b}});
   }
   void finish() {
@@ -115,6 +115,7 @@ void check_ready(fecx_ctx* ctx) {
 
 fecx::ElementArgs element_args(fecx_ctx* ctx, const double* U, const double* V) {
   fecx::ElementArgs a{};
+  a.ebeg = 0;
   a.ne = ctx->ne;
   a.conn = get<int32_t>(ctx, "conn");
   a.eperm = get<int32_t>(ctx, "eperm");
@@ -134,6 +135,8 @@ fecx::ElementArgs element_args(fecx_ctx* ctx, const double* U, const double* V) 
 fecx::ReduceArgs reduce_args(fecx_ctx* ctx) {
   fecx::ReduceArgs r{};
   r.nn = ctx->nn;
+  r.p0 = 0;
+  r.p1 = ctx->nn;
   r.norder = get<int32_t>(ctx, "norder");
   r.nnpe = ctx->nnpe;
   r.vstride = ctx->hp.vstride;
@@ -143,12 +146,27 @@ fecx::ReduceArgs reduce_args(fecx_ctx* ctx) {
   r.voff = get<int32_t>(ctx, "voff");
   r.vent = get<int32_t>(ctx, "vent");
   r.vslot = get<uint8_t>(ctx, "vslot");
+  r.kel_off = get<int16_t>(ctx, "kel_off");
   r.Kel = get<float>(ctx, "Kel");
   r.Rel = get<float>(ctx, "Rel");
   return r;
 }
 
-// runs the element kernel for `mode`, then the requested reductions; syncs and checks the flag
+// Number of pipeline chunks of a matrix assembly. One by default: on MI355X the overlapped pieces measured slower
+// than back-to-back launches (hex8 neo-Hookean 96^3: 2.04 ms one piece, 2.15-2.38 ms with 4-32 pieces), the two
+// kernels competing for the same CU resources.
+int pipeline_chunks(fecx_ctx* ctx, int mode) {
+  if (!(mode & fecx::MODE_K)) return 1;
+  return ctx->chunks > 0 ? ctx->chunks : 1;
+}
+
+// Runs the element kernel for `mode`, then the requested reductions; syncs and checks the flag.
+// With more than one chunk, the Morton-ordered row nodes are split into `k` ranges of positions [p_j, p_j+1); element
+// chunk j is the storage range of the elements whose smallest node rank lies in that range (contiguous, elements are
+// stored sorted by that key). A node only meets elements whose key is <= its own rank, so reduction chunk j needs
+// element chunks 0..j: element chunks run in order on stream2, reduction chunk j on the context's stream after an
+// event recorded behind element chunk j. Element chunk j + 1 (arithmetic-bound) overlaps reduction chunk j
+// (memory-bound), which reads element storage written moments earlier.
 void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double* R, double* vals) {
   check_ready(ctx);
   require(U != nullptr, FECX_ERR_ARG, "U is NULL");
@@ -157,16 +175,40 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   zero_int_kernel<<<1, 1, 0, ctx->stream>>>(flag);
   fecx::ElementArgs ea = element_args(ctx, U, V);
   if (ctx->mp.phys == FECX_J2) require(ea.st_old != nullptr, FECX_ERR_STATE, "state not allocated");
-  int rc = 0;
-  prof.run("element", [&] { rc = fecx::launch_element(ctx->kind, ctx->nf, mode, ea, ctx->stream); });
-  require(rc == 0, FECX_ERR_UNSUPPORTED, "no element kernel for this element kind / nf / physics");
-  HIPCHK(hipGetLastError());
   fecx::ReduceArgs ra = reduce_args(ctx);
   if (mode & (fecx::MODE_R | fecx::MODE_ACTION_MF | fecx::MODE_ACTION_ASM)) ra.R = R;
   if (mode & fecx::MODE_K) ra.vals = vals;
-  prof.run("reduce", [&] { rc = fecx::launch_reduce(ctx->nf, ra, ctx->stream); });
-  require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
-  HIPCHK(hipGetLastError());
+  const int k = pipeline_chunks(ctx, mode);
+  hipStream_t es = ctx->stream;
+  if (k > 1) {
+    if (!ctx->stream2) HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+    while ((int)ctx->events.size() < k + 1) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      ctx->events.push_back(e);
+    }
+    es = ctx->stream2;
+    HIPCHK(hipEventRecord(ctx->events[k], ctx->stream));  // element storage is free once earlier work is done
+    HIPCHK(hipStreamWaitEvent(es, ctx->events[k], 0));
+  }
+  for (int j = 0; j < k; ++j) {
+    const int64_t p0 = ctx->nn * j / k, p1 = ctx->nn * (j + 1) / k;
+    ea.ebeg = ctx->hp.ekey[p0];
+    ea.ne = ctx->hp.ekey[p1];
+    int rc = 0;
+    prof.run("element", es, [&] { rc = fecx::launch_element(ctx->kind, ctx->nf, mode, ea, es); });
+    require(rc == 0, FECX_ERR_UNSUPPORTED, "no element kernel for this element kind / nf / physics");
+    HIPCHK(hipGetLastError());
+    if (k > 1) {
+      HIPCHK(hipEventRecord(ctx->events[j], es));
+      HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->events[j], 0));
+    }
+    ra.p0 = p0;
+    ra.p1 = p1;
+    prof.run("reduce", ctx->stream, [&] { rc = fecx::launch_reduce(ctx->nf, ra, ctx->stream); });
+    require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   prof.finish();
   int hflag = 0;
@@ -199,6 +241,8 @@ int fecx_destroy(fecx_ctx* ctx) {
   if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
   for (auto& kv : ctx->buf)
     if (kv.second.p) (void)hipFree(kv.second.p);
+  for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   delete ctx;
   return FECX_OK;
 }
@@ -284,7 +328,8 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
   return guard(ctx, [&] {
     require(ctx->has_dofs, FECX_ERR_STATE, "fecx_set_dofs has not been called");
     ctx->pairs = fecx::make_pairs(ctx->nnpe);
-    require((ctx->ne * ctx->pairs.npairs) < (int64_t(1) << 31), FECX_ERR_ARG, "element storage exceeds 2^31 blocks");
+    require((ctx->ne * ctx->pairs.npairs * ctx->nf * ctx->nf) < (int64_t(1) << 31), FECX_ERR_ARG,
+            "element storage exceeds 2^31 words");
     fecx::build_host_pattern(ctx->hp, ctx->nn, ctx->ne, ctx->nnpe, ctx->conn0.data(), ctx->nf, ctx->node_free,
                              ctx->pairs, ctx->nd, ctx->coords.data());
     const int nf = ctx->nf;
@@ -302,6 +347,17 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
     upload(ctx, "voff", hp.voff);
     upload(ctx, "vent", hp.vent);
     upload(ctx, "vslot", hp.vslot);
+    {  // staged row word j = (b, c) of local node a -> word inside the symmetric element storage (K_ba transposed)
+      const int nnpe = ctx->nnpe, bw = nnpe * nf * nf;
+      std::vector<int16_t> off((size_t)nnpe * bw);
+      for (int la = 0; la < nnpe; ++la)
+        for (int j = 0; j < bw; ++j) {
+          const int b = j / (nf * nf), c = j % (nf * nf);
+          const fecx::PairRef pr = fecx::pair_ref(nnpe, la, b);
+          off[(size_t)la * bw + j] = (int16_t)(pr.slot * nf * nf + (pr.ab ? c : (c % nf) * nf + c / nf));
+        }
+      upload(ctx, "kel_off", off);
+    }
     dev(ctx, "Rel", sizeof(float) * ctx->ne * ctx->nnpe * nf);
     dev(ctx, "Kel", sizeof(float) * ctx->ne * ctx->pairs.npairs * nf * nf);
     ctx->has_pattern = true;
@@ -485,6 +541,13 @@ int fecx_extract_field_unknowns(fecx_ctx* ctx, const double* U, double* Uu) {
                                  ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int fecx_set_chunks(fecx_ctx* ctx, int chunks) {
+  return guard(ctx, [&] {
+    require(chunks >= 0 && chunks <= 1024, FECX_ERR_ARG, "chunks must be in [0, 1024]");
+    ctx->chunks = chunks;
   });
 }
 

@@ -72,6 +72,7 @@ struct HostPattern {
   std::vector<int32_t> norder;      // processing position -> node (Morton order)
   std::vector<int32_t> eperm;       // storage position -> element
   std::vector<int32_t> epos;        // element -> storage position
+  std::vector<int64_t> ekey;        // nn + 1: storage positions of the elements whose smallest node rank is < p
   std::vector<int32_t> voff;        // nn + 1 : node -> incidences
   std::vector<int32_t> vent;        // storage position * nnpe + a, ascending element id per node
   std::vector<uint8_t> vslot;       // incidences x vstride bytes: column slot per local node b
@@ -91,7 +92,7 @@ struct MatParams {
 enum Mode { MODE_R = 1, MODE_K = 2, MODE_ACTION_MF = 4, MODE_ACTION_ASM = 8 };
 
 struct ElementArgs {
-  int64_t ne;
+  int64_t ebeg, ne;         // element storage positions [ebeg, ne)
   const int32_t* conn;      // ne x nnpe (0-based), in element storage order
   const int32_t* eperm;     // storage position -> element id (quadrature state index)
   const float* xhi;         // nd x nn
@@ -111,6 +112,7 @@ int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t
 
 struct ReduceArgs {
   int64_t nn;
+  int64_t p0, p1;           // processing positions [p0, p1) of this launch
   const int32_t* norder;
   int nnpe, vstride, maxdeg;
   const int32_t* free_rank;
@@ -118,6 +120,7 @@ struct ReduceArgs {
   const int32_t* voff;
   const int32_t* vent;
   const uint8_t* vslot;
+  const int16_t* kel_off;   // nnpe x (nnpe*nf*nf): word of (local node a, staged word j) in an element's storage
   const float* Kel;
   const float* Rel;
   double* vals;             // CSR values, or nullptr
@@ -169,6 +172,10 @@ struct fecx_ctx {
   int ns = 0;
   // device buffers
   std::map<std::string, DevBuf> buf;
+  // pipelined assembly: element chunks on stream2, reduction chunks on the context's stream
+  int chunks = 0;  // 0: chosen from the element storage size
+  hipStream_t stream2 = nullptr;
+  std::vector<hipEvent_t> events;
   // profiling
   bool profiling = false;
   std::vector<std::string> prof_names;

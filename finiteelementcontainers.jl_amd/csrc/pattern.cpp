@@ -120,6 +120,7 @@ void build_host_pattern(HostPattern& hp, int64_t nn, int64_t ne, int nnpe, const
     std::vector<int64_t> cnt(nn + 1, 0);
     for (int64_t el = 0; el < ne; ++el) cnt[key[el] + 1]++;
     for (int64_t n = 0; n < nn; ++n) cnt[n + 1] += cnt[n];
+    hp.ekey = cnt;
     hp.eperm.resize(ne);
     hp.epos.resize(ne);
     for (int64_t el = 0; el < ne; ++el) {

@@ -55,15 +55,11 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
   __shared__ float s_r[NPW][CH][NF];
   __shared__ float s_blk[NPW][CH][BW];
   __shared__ int s_maxpass;
-  for (int i = threadIdx.x; i < NNPE * BW; i += RED_THREADS) {
-    const int la = i / BW, j = i % BW, b = j / (NF * NF), c = j % (NF * NF);
-    const PairRef pr = pair_ref(NNPE, la, b);
-    s_off[la][j] = (int16_t)(pr.slot * NF * NF + (pr.ab ? c : (c % NF) * NF + c / NF));
-  }
+  for (int i = threadIdx.x; i < NNPE * BW; i += RED_THREADS) (&s_off[0][0])[i] = a.kel_off[i];
   const int64_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
-  const int64_t pos = chunk * NPW + grp;
-  const bool valid = pos < a.nn;
+  const int64_t pos = a.p0 + chunk * NPW + grp;
+  const bool valid = pos < a.p1;
   const int64_t n = valid ? a.norder[pos] : 0;
   int32_t c0 = 0, ninc = 0;
   if (valid) { c0 = a.voff[n]; ninc = a.voff[n + 1] - c0; }
@@ -104,10 +100,14 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
     __syncthreads();
     if (r >= 0) {
       for (int t = 0; t < cnt; ++t) {
-        const int32_t v = s_v[grp][t];
-        const float* src = a.Kel + (int64_t)(v / NNPE) * EW;
-        const int la = v % NNPE;
-        for (int j = lane; j < BW; j += LPR) s_blk[grp][t][j] = src[s_off[la][j]];
+        const uint32_t v = (uint32_t)s_v[grp][t];
+        const uint32_t ebase = (v / NNPE) * (uint32_t)EW;  // element storage is limited to 2^31 words
+        const int la = (int)(v % NNPE);
+#pragma unroll
+        for (int j0 = 0; j0 < BW; j0 += LPR) {
+          const int j = j0 + lane;
+          if (j < BW) s_blk[grp][t][j] = a.Kel[ebase + (uint32_t)s_off[la][j]];
+        }
       }
     }
     __syncthreads();
@@ -156,7 +156,7 @@ static int launch_lpr(const ReduceArgs& a, hipStream_t s) {
   if (a.maxdeg > 2 * lpr) return -1;
   const int sw = a.maxdeg <= lpr ? 1 : 2;
   const int64_t npw = RED_THREADS / lpr;
-  int64_t grid = (a.nn + npw - 1) / npw;
+  int64_t grid = (a.p1 - a.p0 + npw - 1) / npw;
   grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
   const unsigned g = (unsigned)grid;
   if (lpr == 16 && sw == 1) reduce_rows_kernel<NF, NNPE, 16, 1><<<g, RED_THREADS, 0, s>>>(a);
@@ -168,7 +168,7 @@ static int launch_lpr(const ReduceArgs& a, hipStream_t s) {
 }
 
 int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s) {
-  if (a.nn == 0) return 0;
+  if (a.p1 <= a.p0) return 0;
   if (nf == 1) {
     if (a.nnpe == 4) return launch_lpr<1, 4>(a, s);
     if (a.nnpe == 8) return launch_lpr<1, 8>(a, s);

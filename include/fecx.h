@@ -163,7 +163,15 @@ int fecx_extract_field_unknowns(fecx_ctx* ctx, const double* U, double* Uu);
 
 /* ---------------------------------------------------------------- instrumentation */
 
-/* When on, every kernel launch is bracketed by HIP events on the context's stream and its
+/* Pipelining of matrix assemblies (no counterpart in the reference, whose CPU and
+ * KernelAbstractions paths run the element loop and sparse! one after the other,
+ * src/assemblers/SparsityPatterns.jl:301-329): the element kernel and the reduction run in
+ * `chunks` Morton-ordered pieces on two streams, element piece j+1 overlapping reduction piece j.
+ * 0 (the default) and 1 run them back to back as one piece. Results do not depend on it (same
+ * sums, same order). */
+int fecx_set_chunks(fecx_ctx* ctx, int chunks);
+
+/* When on, every kernel launch is bracketed by HIP events on the stream it runs on and its
  * duration accumulated per kernel name (the reference's only instrumentation is TimerOutputs
  * sections, src/Solvers.jl:133-152). */
 int fecx_set_profiling(fecx_ctx* ctx, int on);

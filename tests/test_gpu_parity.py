@@ -146,3 +146,21 @@ def test_numeric_error_flag(fec, gpu):
     with pytest.raises(fec.FECXError) as ei:
         fec.assemble_vector_(asm, fec.residual, Uu, p)
     assert ei.value.code == -5
+
+
+@pytest.mark.parametrize("name", ["neohookean_hex8", "poisson_tet10"])
+def test_pipelined_chunks_bit_identical(fec, gpu, name):
+    """Pipelined matrix assembly (element and reduction pieces on two streams) gives the same bits as one piece."""
+    print(f"[gpu chunks] {name}", flush=True)
+    wl = _workload(fec, name)
+    asm, p = _setup(fec, wl, gpu)
+    U = wl.U.copy()
+    U[asm.dof.dirichlet_dofs - 1] = 0.0
+    Uu = torch.from_numpy(U[asm.dof.unknown_dofs - 1]).to(gpu)
+    out = []
+    for chunks in (1, 5, 1000):
+        fec.set_chunks(asm, chunks)
+        fec.assemble_vector_and_stiffness_(asm, Uu, p)
+        out.append((fec.residual(asm).cpu().numpy().copy(), asm.stiffness_storage.cpu().numpy().copy()))
+    for R, K in out[1:]:
+        assert np.array_equal(R, out[0][0]) and np.array_equal(K, out[0][1])
