@@ -69,6 +69,19 @@ def kernel_bytes(wl, asm, with_r=True, with_k=True):
     }
 
 
+def pmc_traffic(workload, kernel):
+    """HBM bytes per launch (read + written) of `kernel` at `workload`'s full size, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary kept in profiles/ (scaled by a copy-kernel calibration there), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 5 and f[0] == "TRAFFIC" and f[1] == workload and f[2] == kernel:
+            return int(f[3]) + int(f[4])
+    return None
+
+
 def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
@@ -177,6 +190,7 @@ def main():
     ach = kernels[dom]["gbs"]
     log("[bench] kernels: " + ", ".join(f"{k} {v['ms_per_step']:.3f} ms ({v['launches_per_step']:.0f} launches) {v.get('gbs', 0):.0f} GB/s" for k, v in kernels.items()))
 
+    traffic = pmc_traffic(name, dom) if n == n_full else None
     out = {
         "metric": METRIC,
         "value": wl.num_elements / dt,
@@ -194,7 +208,7 @@ def main():
                    "elements": int(wl.num_elements), "nnz": int(asm.nnz), "step": "residual + CSR Jacobian",
                    "element_math": "fp32", "reductions": "fp64", "chunks": max(1, args.chunks)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None},
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic},
         "kernels": kernels,
     }
     if name in SURVEY_BYTES:
