@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfecx.so")
 
 QUAD4, HEX8, TET10 = 0, 1, 2
-POISSON, LINEAR_ELASTIC, NEOHOOKEAN, J2, J2_SURROGATE = 0, 1, 2, 3, 4
+POISSON, LINEAR_ELASTIC, NEOHOOKEAN, J2, J2_SURROGATE, NH_SURROGATE = 0, 1, 2, 3, 4, 5
+MLP_J2, MLP_NH = 0, 1
 
 ERRORS = {
     -1: "FECX_ERR_ARG",
@@ -53,6 +54,24 @@ _SIGS = {
     "fecx_update_field_unknowns": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_update_field_dirichlet": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_extract_field_unknowns": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_surrogate_init": [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64],
+    "fecx_surrogate_num_params": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)],
+    "fecx_surrogate_get_params": [ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_surrogate_set_params": [ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_surrogate_set_scaling": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_surrogate_get_scaling": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_surrogate_fit_scaling": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64],
+    "fecx_surrogate_forward": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p],
+    "fecx_surrogate_gradient": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                ctypes.POINTER(ctypes.c_double), ctypes.c_void_p],
+    "fecx_surrogate_train": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                             ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p],
+    "fecx_surrogate_set_tolerance": [ctypes.c_void_p, ctypes.c_float],
+    "fecx_surrogate_fallbacks": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
+    "fecx_material_update": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p],
+    "fecx_qp_gradients": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_j2_generate_samples": [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
+                                 ctypes.c_void_p, ctypes.c_void_p],
     "fecx_set_chunks": [ctypes.c_void_p, ctypes.c_int],
     "fecx_set_profiling": [ctypes.c_void_p, ctypes.c_int],
     "fecx_profile_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],

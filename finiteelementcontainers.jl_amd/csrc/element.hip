@@ -14,371 +14,9 @@
 // instead of the 45 of a dense symmetric 9 x 9 tangent.
 // Outputs are FP32 element-private slots in element storage order; the FP64 sums into the residual field and
 // the CSR values happen in the reduction kernel (reduce.hip), through the precomputed inverse COO->CSR map.
-#include "fecx_internal.h"
+#include "fem_device.h"
 
 namespace fecx {
-
-template <int KIND>
-struct ET;
-template <>
-struct ET<FECX_QUAD4> {
-  static constexpr int NNPE = 4, ND = 2, NQ = 4, G = 4, NPAIRS = 10, MAXP = 3;
-};
-template <>
-struct ET<FECX_HEX8> {
-  static constexpr int NNPE = 8, ND = 3, NQ = 8, G = 8, NPAIRS = 36, MAXP = 5;
-};
-template <>
-struct ET<FECX_TET10> {
-  static constexpr int NNPE = 10, ND = 3, NQ = 4, G = 16, NPAIRS = 55, MAXP = 6;
-};
-
-__host__ __device__ constexpr int lane_cnt(int nnpe, int a) {
-  return a >= nnpe ? 0 : (nnpe - 1) / 2 + 1 + ((nnpe % 2 == 0 && a < nnpe / 2) ? 1 : 0);
-}
-
-// ------------------------------------------------------------------ reference elements (FP32)
-template <int KIND>
-__device__ __forceinline__ float qweight() {
-  if constexpr (KIND == FECX_TET10) return 1.0f / 24.0f;
-  else return 1.0f;
-}
-
-// Quadrature point q in reference coordinates: Gauss-Legendre 2 points per direction for quad4/hex8 (xi fastest),
-// the 4-point degree-2 rule (a=(5-sqrt5)/20, b=(5+3sqrt5)/20, weights 1/24) for tet10.
-template <int KIND>
-__device__ __forceinline__ void qpoint(int q, float* xi) {
-  constexpr float g = 0.57735026918962576f;
-  if constexpr (KIND == FECX_QUAD4) {
-    xi[0] = (q & 1) ? g : -g; xi[1] = (q & 2) ? g : -g;
-  } else if constexpr (KIND == FECX_HEX8) {
-    xi[0] = (q & 1) ? g : -g; xi[1] = (q & 2) ? g : -g; xi[2] = (q & 4) ? g : -g;
-  } else {
-    constexpr float qa = 0.13819660112501051f, qb = 0.58541019662496845f;
-    xi[0] = q == 1 ? qb : qa; xi[1] = q == 2 ? qb : qa; xi[2] = q == 3 ? qb : qa;
-  }
-}
-
-// N_a and dN_a/dxi of one local node a (quad4/hex8 in the structured mesh's local order, tet10 in Exodus TETRA10
-// order: vertices, then edges 12, 23, 31, 14, 24, 34).
-template <int KIND>
-__device__ __forceinline__ void shape_node(int a, const float* xi, float& N, float* dN) {
-  if constexpr (KIND == FECX_QUAD4) {
-    const float sx = ((a & 3) == 1 || (a & 3) == 2) ? 1.0f : -1.0f, sy = (a & 2) ? 1.0f : -1.0f;
-    const float fx = 1 + sx * xi[0], fy = 1 + sy * xi[1];
-    N = 0.25f * fx * fy;
-    dN[0] = 0.25f * sx * fy;
-    dN[1] = 0.25f * fx * sy;
-  } else if constexpr (KIND == FECX_HEX8) {
-    const float sx = ((a & 3) == 1 || (a & 3) == 2) ? 1.0f : -1.0f, sy = (a & 2) ? 1.0f : -1.0f;
-    const float sz = (a & 4) ? 1.0f : -1.0f;
-    const float fx = 1 + sx * xi[0], fy = 1 + sy * xi[1], fz = 1 + sz * xi[2];
-    N = 0.125f * fx * fy * fz;
-    dN[0] = 0.125f * sx * fy * fz;
-    dN[1] = 0.125f * fx * sy * fz;
-    dN[2] = 0.125f * fx * fy * sz;
-  } else {
-    // barycentric L0 = 1 - x - y - z, L1 = x, L2 = y, L3 = z
-    auto L = [&](int v) { return v == 0 ? 1.0f - xi[0] - xi[1] - xi[2] : xi[v - 1]; };
-    auto dL = [&](int v, int j) { return v == 0 ? -1.0f : (v - 1 == j ? 1.0f : 0.0f); };
-    if (a < 4) {
-      const float l = L(a);
-      N = l * (2 * l - 1);
-#pragma unroll
-      for (int j = 0; j < 3; ++j) dN[j] = (4 * l - 1) * dL(a, j);
-    } else {
-      // edges 4:(0,1) 5:(1,2) 6:(2,0) 7:(0,3) 8:(1,3) 9:(2,3)
-      const int m = a - 4;
-      const int p = m < 3 ? m : m - 3;
-      const int r = m < 3 ? (m + 1) % 3 : 3;
-      const float lp = L(p), lr = L(r);
-      N = 4 * lp * lr;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) dN[j] = 4 * (dL(p, j) * lr + lp * dL(r, j));
-    }
-  }
-}
-
-// ------------------------------------------------------------------ materials (FP32)
-// Each material provides:
-//   NC                         floats of compact tangent data per quadrature point
-//   eval(mp, H, so, sn, P, C)  stress P (NF x ND) and tangent data C from the field gradient H (and state)
-//   prep(mp, C, g, v)          per-node vector v (NV floats, v[0..ND) = g) from the shape gradient g: K_ab depends
-//                              on the pair only through v_a, v_b
-//   NA, side(mp, C, w, va, s)  the a-side factors s (NA floats) of one quadrature point, weight folded in
-//   pair(s, vb, K)             K += w * (g_a . A . g_b) as an NF x NF block
-//   dP(mp, C, Gv, out)         out = A : Gv (matrix-free action)
-template <int PHYS>
-struct Mat;
-
-// -div(grad u) = f (test/poisson/TestPoissonCommon.jl:75-127): P = grad u, A = I
-template <>
-struct Mat<FECX_POISSON> {
-  static constexpr int NF = 1, NC = 1, NV = 3, NS = 0, S0 = 0;  // side scalars: C[S0 .. S0+NS)
-  template <int ND>
-  __device__ static void eval(const MatParams&, const float* H, const float*, float*, float* P, float*, bool&) {
-#pragma unroll
-    for (int j = 0; j < ND; ++j) P[j] = H[j];
-  }
-  template <int ND>
-  __device__ static void prep(const MatParams&, const float*, const float* g, float* v) {
-#pragma unroll
-    for (int j = 0; j < ND; ++j) v[j] = g[j];
-  }
-  static constexpr int NA = 3;
-  template <int ND>
-  __device__ static void side(const MatParams&, const float*, float w, const float* va, float* sa) {
-#pragma unroll
-    for (int j = 0; j < ND; ++j) sa[j] = w * va[j];
-  }
-  template <int ND>
-  __device__ static void pair(const float* sa, const float* vb, float* K) {
-#pragma unroll
-    for (int j = 0; j < ND; ++j) K[0] = fmaf(sa[j], vb[j], K[0]);
-  }
-  template <int ND>
-  __device__ static void dP(const MatParams&, const float*, const float* Gv, float* out) {
-#pragma unroll
-    for (int j = 0; j < ND; ++j) out[j] = Gv[j];
-  }
-};
-
-// Small-strain linear elasticity of the reference's test Mechanics physics, psi = K/2 tr(eps)^2 + G dev eps : dev eps
-// (test/mechanics/TestMechanicsCommon.jl:14-20): A = (K - 2G/3) I(x)I + G (d_ik d_jl + d_il d_jk).
-template <>
-struct Mat<FECX_LINEAR_ELASTIC> {
-  static constexpr int NF = 3, NC = 1, NV = 3, NS = 0, S0 = 0;
-  template <int ND>
-  __device__ static void eval(const MatParams& mp, const float* H, const float*, float*, float* P, float*, bool&) {
-    const float K = mp.p[1], G = mp.p[2], lm = K - 2.0f * G / 3.0f;
-    const float tr = H[0] + H[4] + H[8];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) P[i * 3 + j] = G * (H[i * 3 + j] + H[j * 3 + i]) + (i == j ? lm * tr : 0.0f);
-  }
-  template <int ND>
-  __device__ static void prep(const MatParams&, const float*, const float* g, float* v) {
-    v[0] = g[0]; v[1] = g[1]; v[2] = g[2];
-  }
-  static constexpr int NA = 6;  // {w lm g_a, w G g_a}
-  template <int ND>
-  __device__ static void side(const MatParams& mp, const float*, float w, const float* ga, float* sa) {
-    const float G = mp.p[2], lm = mp.p[1] - 2.0f * G / 3.0f;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) { sa[j] = w * lm * ga[j]; sa[3 + j] = w * G * ga[j]; }
-  }
-  template <int ND>
-  __device__ static void pair(const float* sa, const float* gb, float* Kb) {
-    const float gg = fmaf(sa[3], gb[0], fmaf(sa[4], gb[1], sa[5] * gb[2]));
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float t = fmaf(sa[i], gb[k], Kb[i * 3 + k]);
-        t = fmaf(gb[i], sa[3 + k], t);
-        Kb[i * 3 + k] = i == k ? t + gg : t;
-      }
-  }
-  template <int ND>
-  __device__ static void dP(const MatParams& mp, const float*, const float* Gv, float* out) {
-    const float G = mp.p[2], lm = mp.p[1] - 2.0f * G / 3.0f;
-    const float tr = Gv[0] + Gv[4] + Gv[8];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) out[i * 3 + j] = G * (Gv[i * 3 + j] + Gv[j * 3 + i]) + (i == j ? lm * tr : 0.0f);
-  }
-};
-
-// Compressible neo-Hookean, psi = mu/2 (I1 - 3) - mu ln J + lambda/2 (ln J)^2, written in the displacement gradient H
-// so that no term cancels for small H: J - 1 = tr H + (tr^2 H - tr H^2)/2 + det H, ln J = log1p(J - 1),
-// P = mu H + mu H^T F^-T + lambda ln J F^-T.
-// A_iJkL = mu d_ik d_JL + (mu - lambda ln J) Finv_Li Finv_Jk + lambda Finv_Ji Finv_Lk, so with h = F^-1 g
-// (h_i = sum_J F^-T_iJ g_J): K_ab = mu (g_a . g_b) I + (mu - lambda ln J) h_b (x) h_a + lambda h_a (x) h_b.
-// C = {F^-T (9, row-major), mu - lambda ln J}.
-template <>
-struct Mat<FECX_NEOHOOKEAN> {
-  static constexpr int NF = 3, NC = 10, NV = 6, NS = 1, S0 = 9;
-  template <int ND>
-  __device__ static void eval(const MatParams& mp, const float* H, const float*, float*, float* P, float* C, bool& bad) {
-    const float lam = mp.p[0], mu = mp.p[1];
-    const float tr = H[0] + H[4] + H[8];
-    const float trH2 = H[0] * H[0] + H[4] * H[4] + H[8] * H[8] + 2.0f * (H[1] * H[3] + H[2] * H[6] + H[5] * H[7]);
-    const float detH = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
-                       H[2] * (H[3] * H[7] - H[4] * H[6]);
-    const float Jm1 = tr + 0.5f * (tr * tr - trH2) + detH;
-    const float J = 1.0f + Jm1;
-    if (!(J > 0.0f)) bad = true;
-    const float lnJ = log1pf(Jm1);
-    const float F[9] = {1 + H[0], H[1], H[2], H[3], 1 + H[4], H[5], H[6], H[7], 1 + H[8]};
-    const float iJ = 1.0f / J;
-    float FiT[9];  // cofactor(F) / J
-    FiT[0] = (F[4] * F[8] - F[5] * F[7]) * iJ; FiT[1] = (F[5] * F[6] - F[3] * F[8]) * iJ;
-    FiT[2] = (F[3] * F[7] - F[4] * F[6]) * iJ; FiT[3] = (F[2] * F[7] - F[1] * F[8]) * iJ;
-    FiT[4] = (F[0] * F[8] - F[2] * F[6]) * iJ; FiT[5] = (F[1] * F[6] - F[0] * F[7]) * iJ;
-    FiT[6] = (F[1] * F[5] - F[2] * F[4]) * iJ; FiT[7] = (F[2] * F[3] - F[0] * F[5]) * iJ;
-    FiT[8] = (F[0] * F[4] - F[1] * F[3]) * iJ;
-    if (P) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          const float htf = H[0 * 3 + i] * FiT[0 * 3 + j] + H[1 * 3 + i] * FiT[1 * 3 + j] + H[2 * 3 + i] * FiT[2 * 3 + j];
-          P[i * 3 + j] = mu * (H[i * 3 + j] + htf) + lam * lnJ * FiT[i * 3 + j];
-        }
-    }
-    if (C) {
-#pragma unroll
-      for (int i = 0; i < 9; ++i) C[i] = FiT[i];
-      C[9] = mu - lam * lnJ;
-    }
-  }
-  template <int ND>
-  __device__ static void prep(const MatParams&, const float* C, const float* g, float* v) {
-    v[0] = g[0]; v[1] = g[1]; v[2] = g[2];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) v[3 + i] = C[i * 3 + 0] * g[0] + C[i * 3 + 1] * g[1] + C[i * 3 + 2] * g[2];
-  }
-  static constexpr int NA = 9;  // {w mu g_a, w c1 h_a, w lambda h_a}
-  template <int ND>
-  __device__ static void side(const MatParams& mp, const float* C, float w, const float* va, float* sa) {
-    const float wl = w * mp.p[0], wm = w * mp.p[1], wc = w * C[9];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) { sa[j] = wm * va[j]; sa[3 + j] = wc * va[3 + j]; sa[6 + j] = wl * va[3 + j]; }
-  }
-  template <int ND>
-  __device__ static void pair(const float* sa, const float* vb, float* Kb) {
-    const float gg = fmaf(sa[0], vb[0], fmaf(sa[1], vb[1], sa[2] * vb[2]));
-    const float* hb = vb + 3;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float t = fmaf(hb[i], sa[3 + k], Kb[i * 3 + k]);
-        t = fmaf(sa[6 + i], hb[k], t);
-        Kb[i * 3 + k] = i == k ? t + gg : t;
-      }
-  }
-  template <int ND>
-  __device__ static void dP(const MatParams& mp, const float* C, const float* Gv, float* out) {
-    // mu Gv + c1 F^-T Gv^T F^-T + lambda (F^-T : Gv) F^-T
-    const float lam = mp.p[0], mu = mp.p[1], c1 = C[9];
-    float M[9];  // Gv^T F^-T
-#pragma unroll
-    for (int l = 0; l < 3; ++l)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) M[l * 3 + j] = Gv[0 * 3 + l] * C[0 * 3 + j] + Gv[1 * 3 + l] * C[1 * 3 + j] + Gv[2 * 3 + l] * C[2 * 3 + j];
-    float tr = 0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) tr += C[i] * Gv[i];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        out[i * 3 + j] = mu * Gv[i * 3 + j] + c1 * (C[i * 3 + 0] * M[0 * 3 + j] + C[i * 3 + 1] * M[1 * 3 + j] + C[i * 3 + 2] * M[2 * 3 + j]) +
-                         lam * tr * C[i * 3 + j];
-  }
-};
-
-// J2 small-strain plasticity with linear isotropic hardening, one-step radial return and its consistent tangent
-// (Simo & Hughes, Computational Inelasticity, Box 3.1-3.2). State (7): plastic strain in SymmetricTensor order
-// (11,21,31,22,32,33), then equivalent plastic strain. Tangent C = kappa I(x)I + 2 mu theta (I_sym - I(x)I/3)
-// - 2 mu thetabar n(x)n, so with m = n g: K_ab = b1 g_a (x) g_b + b2 ((g_a . g_b) I + g_b (x) g_a) - b3 m_a (x) m_b,
-// b1 = kappa - 2 mu theta / 3, b2 = mu theta, b3 = 2 mu thetabar. C = {n (11,22,33,12,23,13), b1, b2, b3}.
-__device__ __forceinline__ void j2_update(const float* p, const float* H, const float* so, float* sn, float* P, float* C) {
-  const float E = p[0], nu = p[1], sy = p[2], Hh = p[3];
-  const float mu = E / (2 * (1 + nu)), kap = E / (3 * (1 - 2 * nu));
-  const float ep[9] = {so[0], so[1], so[2], so[1], so[3], so[4], so[2], so[4], so[5]};
-  float ee[9];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) ee[i * 3 + j] = 0.5f * (H[i * 3 + j] + H[j * 3 + i]) - ep[i * 3 + j];
-  const float tr = ee[0] + ee[4] + ee[8];
-  float s[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) s[i] = 2 * mu * ee[i];
-  s[0] -= 2 * mu * tr / 3; s[4] -= 2 * mu * tr / 3; s[8] -= 2 * mu * tr / 3;
-  float sn2 = 0;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) sn2 += s[i] * s[i];
-  const float snorm = sqrtf(sn2);
-  const float r23 = 0.81649658092772603f;
-  const float f = snorm - r23 * (sy + Hh * so[6]);
-  const bool plastic = f > 0.0f;
-  const float inv = plastic ? 1.0f / snorm : 0.0f;
-  float nh[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) nh[i] = s[i] * inv;
-  const float dg = plastic ? f / (2 * mu + 2.0f * Hh / 3.0f) : 0.0f;
-  if (P) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) P[i] = s[i] - 2 * mu * dg * nh[i];
-    P[0] += kap * tr; P[4] += kap * tr; P[8] += kap * tr;
-  }
-  if (sn) {
-    sn[0] = ep[0] + dg * nh[0]; sn[1] = ep[3] + dg * nh[3]; sn[2] = ep[6] + dg * nh[6];
-    sn[3] = ep[4] + dg * nh[4]; sn[4] = ep[7] + dg * nh[7]; sn[5] = ep[8] + dg * nh[8];
-    sn[6] = so[6] + r23 * dg;
-  }
-  if (C) {
-    const float th = plastic ? 1.0f - 2 * mu * dg / snorm : 1.0f;
-    const float thb = plastic ? 1.0f / (1.0f + Hh / (3 * mu)) - (1.0f - th) : 0.0f;
-    C[0] = nh[0]; C[1] = nh[4]; C[2] = nh[8]; C[3] = nh[1]; C[4] = nh[5]; C[5] = nh[2];
-    C[6] = kap - 2 * mu * th / 3; C[7] = mu * th; C[8] = 2 * mu * thb;
-  }
-}
-
-template <>
-struct Mat<FECX_J2> {
-  static constexpr int NF = 3, NC = 9, NV = 6, NS = 3, S0 = 6;
-  template <int ND>
-  __device__ static void eval(const MatParams& mp, const float* H, const float* so, float* sn, float* P, float* C, bool&) {
-    j2_update(mp.p, H, so, sn, P, C);
-  }
-  template <int ND>
-  __device__ static void prep(const MatParams&, const float* C, const float* g, float* v) {
-    v[0] = g[0]; v[1] = g[1]; v[2] = g[2];
-    v[3] = C[0] * g[0] + C[3] * g[1] + C[5] * g[2];
-    v[4] = C[3] * g[0] + C[1] * g[1] + C[4] * g[2];
-    v[5] = C[5] * g[0] + C[4] * g[1] + C[2] * g[2];
-  }
-  static constexpr int NA = 9;  // {w b1 g_a, w b2 g_a, -w b3 m_a}
-  template <int ND>
-  __device__ static void side(const MatParams&, const float* C, float w, const float* va, float* sa) {
-    const float w1 = w * C[6], w2 = w * C[7], w3 = -w * C[8];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) { sa[j] = w1 * va[j]; sa[3 + j] = w2 * va[j]; sa[6 + j] = w3 * va[3 + j]; }
-  }
-  template <int ND>
-  __device__ static void pair(const float* sa, const float* vb, float* Kb) {
-    const float gg = fmaf(sa[3], vb[0], fmaf(sa[4], vb[1], sa[5] * vb[2]));
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float t = fmaf(sa[i], vb[k], Kb[i * 3 + k]);
-        t = fmaf(vb[i], sa[3 + k], t);
-        t = fmaf(sa[6 + i], vb[3 + k], t);
-        Kb[i * 3 + k] = i == k ? t + gg : t;
-      }
-  }
-  template <int ND>
-  __device__ static void dP(const MatParams&, const float* C, const float* Gv, float* out) {
-    const float b1 = C[6], b2 = C[7], b3 = C[8];
-    const float tr = Gv[0] + Gv[4] + Gv[8];
-    const float n[9] = {C[0], C[3], C[5], C[3], C[1], C[4], C[5], C[4], C[2]};
-    float nd = 0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) nd += n[i] * Gv[i];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        out[i * 3 + j] = b2 * (Gv[i * 3 + j] + Gv[j * 3 + i]) + (i == j ? b1 * tr : 0.0f) - b3 * nd * n[i * 3 + j];
-  }
-};
 
 // ------------------------------------------------------------------ the kernel
 template <int KIND, int NF, int PHYS, int MODE>
@@ -417,6 +55,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   constexpr int ND = T::ND, NNPE = T::NNPE, NQ = T::NQ, G = T::G, EPW = ELEM_THREADS / G, NDOF = NNPE * NF;
   constexpr int D = NF * ND;
   constexpr bool DO_R = LDS::DO_R, DO_K = LDS::DO_K, DO_MF = LDS::DO_MF, DO_ASM = LDS::DO_ASM;
+  constexpr bool DO_G = (MODE & MODE_QPGRAD) != 0;
   constexpr bool NEED_C = LDS::NEED_C;
   __shared__ LDS lds[EPW];
 
@@ -515,19 +154,32 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
       for (int i = 0; i < NF; ++i)
 #pragma unroll
         for (int j = 0; j < ND; ++j) H[i * ND + j] += L.u[a][i] * g[a][j];
-    float P[D], C[M::NC];
-    float so[7] = {}, sn[7];
-    if constexpr (PHYS == FECX_J2) {
-      const int64_t sidx = ((int64_t)args.eperm[e] * NQ + q) * 7;
+    if constexpr (DO_G) {  // quadrature-point gradients only, element storage order
+      float* o = args.hq + ((int64_t)e * NQ + q) * D;
 #pragma unroll
-      for (int s = 0; s < 7; ++s) so[s] = args.st_old[sidx + s];
-    }
-    M::template eval<ND>(args.mp, H, so, sn, P, C, bad);
-    if constexpr (PHYS == FECX_J2) {
-      if (DO_R && args.st_new) {
+      for (int c = 0; c < D; ++c) o[c] = H[c];
+    } else {
+    float P[D], C[M::NC];
+    if constexpr (M::FROM_QP) {  // stress and tangent data from the batched (surrogate) material pass
+      const float* qm = args.qpmat + ((int64_t)e * NQ + q) * (D + M::NC);
+#pragma unroll
+      for (int c = 0; c < D; ++c) P[c] = qm[c];
+#pragma unroll
+      for (int c = 0; c < M::NC; ++c) C[c] = qm[D + c];
+    } else {
+      float so[7] = {}, sn[7];
+      if constexpr (PHYS == FECX_J2) {
         const int64_t sidx = ((int64_t)args.eperm[e] * NQ + q) * 7;
 #pragma unroll
-        for (int s = 0; s < 7; ++s) args.st_new[sidx + s] = sn[s];
+        for (int s = 0; s < 7; ++s) so[s] = args.st_old[sidx + s];
+      }
+      M::template eval<ND>(args.mp, H, so, sn, P, C, bad);
+      if constexpr (PHYS == FECX_J2) {
+        if (DO_R && args.st_new) {
+          const int64_t sidx = ((int64_t)args.eperm[e] * NQ + q) * 7;
+#pragma unroll
+          for (int s = 0; s < 7; ++s) args.st_new[sidx + s] = sn[s];
+        }
       }
     }
 #pragma unroll
@@ -580,6 +232,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
 #pragma unroll
       for (int c = 0; c < D; ++c) Q.P[c] = out[c];
     }
+    }  // !DO_G
   }
   __syncthreads();
 
@@ -721,6 +374,9 @@ static void launch_modes(int mode, const ElementArgs& a, hipStream_t s) {
     case MODE_R | MODE_K: element_kernel<KIND, NF, PHYS, MODE_R | MODE_K><<<grid, B, 0, s>>>(a); break;
     case MODE_ACTION_MF: element_kernel<KIND, NF, PHYS, MODE_ACTION_MF><<<grid, B, 0, s>>>(a); break;
     case MODE_ACTION_ASM: element_kernel<KIND, NF, PHYS, MODE_ACTION_ASM><<<grid, B, 0, s>>>(a); break;
+    case MODE_QPGRAD:  // gradients do not depend on the physics: one instantiation per (KIND, NF)
+      element_kernel<KIND, NF, NF == 1 ? FECX_POISSON : FECX_NEOHOOKEAN, MODE_QPGRAD><<<grid, B, 0, s>>>(a);
+      break;
     default: break;
   }
 }
@@ -738,10 +394,14 @@ int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t
       if (phys == FECX_LINEAR_ELASTIC) { launch_modes<FECX_HEX8, 3, FECX_LINEAR_ELASTIC>(mode, a, s); return 0; }
       if (phys == FECX_NEOHOOKEAN) { launch_modes<FECX_HEX8, 3, FECX_NEOHOOKEAN>(mode, a, s); return 0; }
       if (phys == FECX_J2) { launch_modes<FECX_HEX8, 3, FECX_J2>(mode, a, s); return 0; }
+      if (phys == FECX_J2_SURROGATE) { launch_modes<FECX_HEX8, 3, FECX_J2_SURROGATE>(mode, a, s); return 0; }
+      if (phys == FECX_NH_SURROGATE) { launch_modes<FECX_HEX8, 3, FECX_NH_SURROGATE>(mode, a, s); return 0; }
     } else {
       if (phys == FECX_LINEAR_ELASTIC) { launch_modes<FECX_TET10, 3, FECX_LINEAR_ELASTIC>(mode, a, s); return 0; }
       if (phys == FECX_NEOHOOKEAN) { launch_modes<FECX_TET10, 3, FECX_NEOHOOKEAN>(mode, a, s); return 0; }
       if (phys == FECX_J2) { launch_modes<FECX_TET10, 3, FECX_J2>(mode, a, s); return 0; }
+      if (phys == FECX_J2_SURROGATE) { launch_modes<FECX_TET10, 3, FECX_J2_SURROGATE>(mode, a, s); return 0; }
+      if (phys == FECX_NH_SURROGATE) { launch_modes<FECX_TET10, 3, FECX_NH_SURROGATE>(mode, a, s); return 0; }
     }
   }
   return -1;

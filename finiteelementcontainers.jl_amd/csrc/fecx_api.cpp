@@ -4,68 +4,11 @@
 #include <cstring>
 #include <stdexcept>
 
-#include "fecx_internal.h"
+#include "api_util.h"
+#include "surrogate.h"
 
 namespace {
-
-struct ApiError : std::runtime_error {
-  int code;
-  ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(x)                                                                                      \
-  do {                                                                                                 \
-    hipError_t e_ = (x);                                                                               \
-    if (e_ != hipSuccess) throw ApiError(FECX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-template <class F>
-int guard(fecx_ctx* ctx, F f) {
-  if (!ctx) return FECX_ERR_ARG;
-  ctx->err.clear();
-  try {
-    if (ctx->device >= 0) HIPCHK(hipSetDevice(ctx->device));
-    f();
-    return FECX_OK;
-  } catch (const ApiError& e) {
-    ctx->err = e.what();
-    return e.code;
-  } catch (const std::exception& e) {
-    ctx->err = e.what();
-    return FECX_ERR_ARG;
-  }
-}
-
-void require(bool c, int code, const std::string& msg) {
-  if (!c) throw ApiError(code, msg);
-}
-
-// device buffers; a host-only context (device -1) allocates nothing
-void* dev(fecx_ctx* ctx, const std::string& name, size_t bytes) {
-  if (ctx->device < 0) return nullptr;
-  DevBuf& b = ctx->buf[name];
-  if (b.bytes != bytes) {
-    if (b.p) HIPCHK(hipFree(b.p));
-    b.p = nullptr;
-    b.bytes = 0;
-    if (bytes) HIPCHK(hipMalloc(&b.p, bytes));
-    b.bytes = bytes;
-  }
-  return b.p;
-}
-
-template <class T>
-T* upload(fecx_ctx* ctx, const std::string& name, const std::vector<T>& v) {
-  T* p = (T*)dev(ctx, name, v.size() * sizeof(T));
-  if (p && !v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  return p;
-}
-
-template <class T>
-T* get(fecx_ctx* ctx, const std::string& name) {
-  auto it = ctx->buf.find(name);
-  return it == ctx->buf.end() ? nullptr : (T*)it->second.p;
-}
+using namespace fecx_api;
 
 // kernel profiling with HIP events on the context's stream
 struct Prof {
@@ -174,7 +117,23 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   int* flag = get<int>(ctx, "flag");
   zero_int_kernel<<<1, 1, 0, ctx->stream>>>(flag);
   fecx::ElementArgs ea = element_args(ctx, U, V);
-  if (ctx->mp.phys == FECX_J2) require(ea.st_old != nullptr, FECX_ERR_STATE, "state not allocated");
+  const bool surrogate = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
+  if (ctx->mp.phys == FECX_J2 || ctx->mp.phys == FECX_J2_SURROGATE)
+    require(ea.st_old != nullptr, FECX_ERR_STATE, "state not allocated");
+  if (surrogate) {
+    // quadrature-point gradients, then the batched surrogate material pass (network, check, plain fallback)
+    require(ctx->sg_arch == (ctx->mp.phys == FECX_J2_SURROGATE ? FECX_MLP_J2 : FECX_MLP_NH), FECX_ERR_STATE,
+            "the surrogate network for this material has not been initialised");
+    const int64_t nqp = ctx->ne * ctx->nq;
+    const int nc = ctx->mp.phys == FECX_J2_SURROGATE ? 9 : 10;
+    ea.hq = (float*)dev_atleast(ctx, "hq", sizeof(float) * nqp * 9);
+    ea.qpmat = (float*)dev_atleast(ctx, "qpmat", sizeof(float) * nqp * (9 + nc));
+    int rc = 0;
+    prof.run("qp_gradients", ctx->stream, [&] { rc = fecx::launch_element(ctx->kind, ctx->nf, fecx::MODE_QPGRAD, ea, ctx->stream); });
+    require(rc == 0, FECX_ERR_UNSUPPORTED, "no gradient kernel for this element kind / nf");
+    prof.run("surrogate_material", ctx->stream, [&] { fecx::surrogate_qp_pass(ctx, ea.hq, (float*)ea.qpmat, ctx->stream); });
+    HIPCHK(hipGetLastError());
+  }
   fecx::ReduceArgs ra = reduce_args(ctx);
   if (mode & (fecx::MODE_R | fecx::MODE_ACTION_MF | fecx::MODE_ACTION_ASM)) ra.R = R;
   if (mode & fecx::MODE_K) ra.vals = vals;
@@ -211,6 +170,7 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   prof.finish();
+  if (surrogate) fecx::surrogate_qp_finish(ctx);
   int hflag = 0;
   HIPCHK(hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost));
   require(hflag == 0, FECX_ERR_NUMERIC, "element kernel met a non-positive Jacobian determinant or a NaN");
@@ -341,6 +301,7 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
       for (int a = 0; a < ctx->nnpe; ++a) conn_s[s * ctx->nnpe + a] = ctx->conn0[(int64_t)hp.eperm[s] * ctx->nnpe + a];
     upload(ctx, "conn", conn_s);
     upload(ctx, "eperm", hp.eperm);
+    upload(ctx, "epos", hp.epos);
     upload(ctx, "norder", hp.norder);
     upload(ctx, "free_rank", hp.free_rank);
     upload(ctx, "bptr", hp.bptr);
@@ -422,7 +383,7 @@ int fecx_set_material(fecx_ctx* ctx, int kind, const double* props, int nprops) 
   return guard(ctx, [&] {
     require(ctx->has_mesh, FECX_ERR_STATE, "fecx_set_mesh has not been called");
     require(nprops >= 0 && nprops <= 8 && (nprops == 0 || props), FECX_ERR_ARG, "bad properties");
-    require(kind >= FECX_POISSON && kind <= FECX_J2, FECX_ERR_UNSUPPORTED, "unsupported physics kind");
+    require(kind >= FECX_POISSON && kind <= FECX_NH_SURROGATE, FECX_ERR_UNSUPPORTED, "unsupported physics kind");
     fecx::MatParams mp{};
     mp.phys = kind;
     for (int i = 0; i < nprops; ++i) mp.p[i] = (float)props[i];
@@ -431,10 +392,12 @@ int fecx_set_material(fecx_ctx* ctx, int kind, const double* props, int nprops) 
       mp.source_kind = (int)props[0];
     }
     if (kind == FECX_LINEAR_ELASTIC) require(nprops >= 3, FECX_ERR_ARG, "linear elastic needs {rho, K, G}");
-    if (kind == FECX_NEOHOOKEAN) require(nprops >= 2, FECX_ERR_ARG, "neo-Hookean needs {lambda, mu}");
-    if (kind == FECX_J2) require(nprops >= 4, FECX_ERR_ARG, "J2 needs {E, nu, sigma_y, H}");
+    const bool nh = kind == FECX_NEOHOOKEAN || kind == FECX_NH_SURROGATE;
+    const bool j2 = kind == FECX_J2 || kind == FECX_J2_SURROGATE;
+    if (nh) require(nprops >= 2, FECX_ERR_ARG, "neo-Hookean needs {lambda, mu}");
+    if (j2) require(nprops >= 4, FECX_ERR_ARG, "J2 needs {E, nu, sigma_y, H}");
     ctx->mp = mp;
-    ctx->ns = kind == FECX_J2 ? 7 : 0;
+    ctx->ns = j2 ? 7 : 0;
     if (ctx->ns && ctx->device >= 0) {
       const size_t bytes = sizeof(float) * ctx->ns * ctx->nq * ctx->ne;
       HIPCHK(hipMemset(dev(ctx, "state_old", bytes), 0, bytes));

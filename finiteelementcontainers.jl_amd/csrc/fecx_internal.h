@@ -89,7 +89,7 @@ struct MatParams {
   float p[8];
 };
 
-enum Mode { MODE_R = 1, MODE_K = 2, MODE_ACTION_MF = 4, MODE_ACTION_ASM = 8 };
+enum Mode { MODE_R = 1, MODE_K = 2, MODE_ACTION_MF = 4, MODE_ACTION_ASM = 8, MODE_QPGRAD = 16 };
 
 struct ElementArgs {
   int64_t ebeg, ne;         // element storage positions [ebeg, ne)
@@ -104,6 +104,8 @@ struct ElementArgs {
   float* Rel;               // ne x ndof   (residual or action output)
   float* Kel;               // ne x npairs x nf*nf
   int* flag;                // error flag (bit 1: bad Jacobian / NaN)
+  float* hq;                // MODE_QPGRAD output: ne x nq x (nf*nd) gradients, element storage order
+  const float* qpmat;       // surrogate materials: ne x nq x (nf*nd + NC) stress and tangent data
   MatParams mp;
 };
 
@@ -172,6 +174,11 @@ struct fecx_ctx {
   int ns = 0;
   // device buffers
   std::map<std::string, DevBuf> buf;
+  // learned surrogate (FECX_MLP_*): parameters and packed weights live in buf ("sg_*")
+  int sg_arch = -1;
+  float sg_tol = 0.02f;
+  int64_t sg_step = 0;                          // Adam steps taken
+  int64_t sg_fallback = 0, sg_points = 0;       // last surrogate pass
   // pipelined assembly: element chunks on stream2, reduction chunks on the context's stream
   int chunks = 0;  // 0: chosen from the element storage size
   hipStream_t stream2 = nullptr;

@@ -52,12 +52,16 @@ typedef struct fecx_ctx fecx_ctx;
  * NEOHOOKEAN      compressible neo-Hookean; props = {lambda, mu}
  * J2              small-strain J2 plasticity, linear isotropic hardening; props = {E, nu, sigma_y, H};
  *                 7 state variables per quadrature point
- * J2_SURROGATE    J2 with the MLP stress/state surrogate and plain-model fallback (fecx_surrogate_*) */
+ * J2_SURROGATE    J2 whose material update is the FECX_MLP_J2 surrogate, checked and with plain-model
+ *                 fallback (fecx_surrogate_*); props as J2
+ * NH_SURROGATE    neo-Hookean whose stress and tangent come from the FECX_MLP_NH surrogate, checked and
+ *                 with plain-model fallback; props as NEOHOOKEAN */
 #define FECX_POISSON 0
 #define FECX_LINEAR_ELASTIC 1
 #define FECX_NEOHOOKEAN 2
 #define FECX_J2 3
 #define FECX_J2_SURROGATE 4
+#define FECX_NH_SURROGATE 5
 
 /* ---------------------------------------------------------------- context */
 
@@ -160,6 +164,67 @@ int fecx_update_field_dirichlet(fecx_ctx* ctx, const double* vals, double* U);
 
 /* Uu = U[unknown_dofs]: extract_field_unknowns! (src/DofManagers.jl:203-213). */
 int fecx_extract_field_unknowns(fecx_ctx* ctx, const double* U, double* Uu);
+
+/* ---------------------------------------------------------------- learned constitutive surrogate
+ * No counterpart in the reference, whose materials are energy functions differentiated by AD inside
+ * the element loop (test/mechanics/TestMechanicsCommon.jl:140-222). A small MLP with GELU hidden
+ * layers runs batched over quadrature points on BF16 MFMA with FP32 accumulation. It is trained on
+ * the GPU by this library's own forward/backward kernels, on samples the plain material model
+ * generates. The plain model stays the fallback wherever the surrogate's output fails a residual
+ * check of the material equations, and it is the judge of the surrogate's error.
+ *   FECX_MLP_J2  13 -> 128 -> 128 -> 128 -> 13. Raw input per sample (16 floats): H (9,
+ *                row-major displacement gradient), state (7: plastic strain in the order
+ *                11,21,31,22,32,33, then eqps). Raw output (13): stress (6, same order), plastic
+ *                strain increment (6), eqps increment. Features: symmetric strain (6), state (7).
+ *   FECX_MLP_NH  9 -> 256 -> 256 -> 15. Raw input H (9). Raw output: P (9, row-major), C^-1 - I
+ *                (6, same symmetric order).
+ * Inputs and outputs are standardized by per-column shift/scale (fecx_surrogate_fit_scaling).
+ * Array arguments of the batched calls are device pointers. */
+#define FECX_MLP_J2 0
+#define FECX_MLP_NH 1
+
+/* (Re)creates the network with He-normal weights (SplitMix64 stream `seed`) and zero biases. */
+int fecx_surrogate_init(fecx_ctx* ctx, int arch, uint64_t seed);
+int fecx_surrogate_num_params(fecx_ctx* ctx, int64_t* nparams);
+/* Canonical FP32 parameter vector (host): per layer W (out x in, row-major), then b. */
+int fecx_surrogate_get_params(fecx_ctx* ctx, float* params);
+int fecx_surrogate_set_params(fecx_ctx* ctx, const float* params);
+/* Standardization (host arrays): features (13 or 9) and raw outputs (13 or 15). */
+int fecx_surrogate_set_scaling(fecx_ctx* ctx, const float* in_shift, const float* in_scale, const float* out_shift,
+                               const float* out_scale);
+int fecx_surrogate_get_scaling(fecx_ctx* ctx, float* in_shift, float* in_scale, float* out_shift, float* out_scale);
+/* Column means and standard deviations of the features of X and of T (n samples, device). */
+int fecx_surrogate_fit_scaling(fecx_ctx* ctx, const float* X, const float* T, int64_t n);
+/* Network output in raw units for n raw inputs. */
+int fecx_surrogate_forward(fecx_ctx* ctx, const float* X, int64_t n, float* Y);
+/* Mean-squared error of the standardized outputs over n samples (host *loss) and its gradient with
+ * respect to the canonical parameters (device, nparams floats); no update. */
+int fecx_surrogate_gradient(fecx_ctx* ctx, const float* X, const float* T, int64_t n, double* loss, float* grad);
+/* Adam training: `epochs` passes over the n samples in shuffled minibatches of `batch` (SplitMix64
+ * stream `seed`), learning rate lr (beta1 0.9, beta2 0.999, eps 1e-8). loss_per_epoch (host,
+ * `epochs` doubles, or NULL) receives the mean minibatch loss of each epoch. */
+int fecx_surrogate_train(fecx_ctx* ctx, const float* X, const float* T, int64_t n, int epochs, int batch, float lr,
+                         uint64_t seed, double* loss_per_epoch);
+/* Residual-check tolerance of the surrogate material update (relative; default 0.02). */
+int fecx_surrogate_set_tolerance(fecx_ctx* ctx, float tol);
+/* Number of quadrature points that fell back to the plain model in the last surrogate pass. */
+int fecx_surrogate_fallbacks(fecx_ctx* ctx, int64_t* n_fallback, int64_t* n_points);
+
+/* Batched material update over n samples with the properties of the context's material:
+ * kind FECX_J2 or FECX_NEOHOOKEAN runs the plain model, FECX_J2_SURROGATE or FECX_NH_SURROGATE the
+ * surrogate with residual check and plain fallback. Raw inputs and outputs as the network's. */
+int fecx_material_update(fecx_ctx* ctx, int kind, const float* in, int64_t n, float* out);
+
+/* Displacement gradients at every quadrature point of the mesh at the full field U (device):
+ * Hq (ne x nq x 9, element-major in the input element order, row-major H). Training samples for
+ * the neo-Hookean surrogate come from here. */
+int fecx_qp_gradients(fecx_ctx* ctx, const double* U, float* Hq);
+
+/* J2 training samples from the plain model: npaths random macroscopic strain paths of `steps`
+ * steps (4 random waypoints, norms max_norm * U(0.2, 1.1)); X (npaths*steps x 16) raw inputs and
+ * T (x 13) raw outputs, device. Uses the context's J2 properties. */
+int fecx_j2_generate_samples(fecx_ctx* ctx, int64_t npaths, int steps, float max_norm, uint64_t seed, float* X,
+                             float* T);
 
 /* ---------------------------------------------------------------- instrumentation */
 

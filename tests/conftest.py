@@ -31,3 +31,13 @@ def gpu():
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
     return torch.device("cuda", 0)
+
+
+@pytest.fixture(scope="session")
+def mlpref():
+    """oracle/mlp.py: the FP64 CPU forward/backward of the surrogate network (test infrastructure)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("fecx_oracle_mlp", os.path.join(ROOT, "oracle", "mlp.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
