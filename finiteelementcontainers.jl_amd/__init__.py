@@ -14,5 +14,7 @@ from .assembler import (  # noqa: F401
     assemble_matrix_action_, assemble_matrix_free_action_, assemble_matrix_free_action_full_, hvp,
     set_chunks, set_profiling, profile, reset_profile,
 )
-from . import rng, workloads, surrogate  # noqa: F401
+from . import rng, workloads, surrogate, solvers  # noqa: F401
+from .solvers import (IterativeLinearSolver, IterativeLinearSolverSettings, NewtonSolver,  # noqa: F401
+                      NewtonSolverSettings)
 from .surrogate import J2Surrogate, NeoHookeanSurrogate, Surrogate  # noqa: F401

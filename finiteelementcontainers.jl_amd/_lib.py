@@ -54,6 +54,13 @@ _SIGS = {
     "fecx_update_field_unknowns": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_update_field_dirichlet": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_extract_field_unknowns": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_csr_spmv": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_csr_diagonal_inverse": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_cg_solve": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                      ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                      ctypes.POINTER(ctypes.c_double)],
+    "fecx_vector_norm": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)],
+    "fecx_vector_axpy": [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_surrogate_init": [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64],
     "fecx_surrogate_num_params": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)],
     "fecx_surrogate_get_params": [ctypes.c_void_p, ctypes.c_void_p],

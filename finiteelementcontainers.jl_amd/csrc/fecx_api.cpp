@@ -305,6 +305,7 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
     upload(ctx, "norder", hp.norder);
     upload(ctx, "free_rank", hp.free_rank);
     upload(ctx, "bptr", hp.bptr);
+    upload(ctx, "bcol", hp.bcol);
     upload(ctx, "voff", hp.voff);
     upload(ctx, "vent", hp.vent);
     upload(ctx, "vslot", hp.vslot);

@@ -1,0 +1,155 @@
+"""Newton with conjugate gradients on the GPU: the caller of the assembly path (src/Solvers.jl:95-220).
+
+IterativeLinearSolver.solve(Uu, p) assembles the residual and the CSR Jacobian in one pass, solves K dU = R with the
+library's FP64 CG on the device (Jacobi-preconditioned; the reference calls Krylov.jl's cg with the identity
+preconditioner) and updates Uu -= dU, as solve!(::IterativeLinearSolver) does (src/Solvers.jl:128-153).
+NewtonSolver.solve repeats that with the reference's convergence rule (src/Solvers.jl:186-220): stop when
+|dU| < abs_increment_tol or |R| < abs_residual_tol or |R| / |R_1| < rel_residual_tol, at most max_iters times.
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .assembler import (_ptr, assemble_vector_and_stiffness_, create_unknowns)
+
+EPS = float(np.finfo(np.float64).eps)
+
+
+@dataclass
+class IterativeLinearSolverSettings:
+    rtol: float = EPS ** 0.5   # Krylov.jl cg defaults: ||r|| <= atol + rtol ||b||
+    atol: float = EPS ** 0.5
+    max_iters: int = 10000
+    preconditioner: str = "jacobi"   # or "none" (the reference's `I`)
+
+
+class IterativeLinearSolver:
+    def __init__(self, assembler, solver_sym="cg", settings=None):
+        if solver_sym != "cg":
+            raise ValueError("only :cg runs on the GPU")
+        self.assembler = assembler
+        self.settings = settings or IterativeLinearSolverSettings()
+        self.dUu = create_unknowns(assembler)
+        self._x = create_unknowns(assembler)
+        self.last_iters = 0
+        self.last_resnorm = 0.0
+
+    def cg(self, vals, b, x):
+        """x <- K^-1 b by CG from the given x; returns (iterations, final residual norm)."""
+        s = self.settings
+        it, res = ctypes.c_int(0), ctypes.c_double(0.0)
+        self.assembler.ctx.call("fecx_cg_solve", _ptr(vals), _ptr(b), _ptr(x), ctypes.c_double(s.rtol),
+                                ctypes.c_double(s.atol), ctypes.c_int(s.max_iters),
+                                ctypes.c_int(1 if s.preconditioner == "jacobi" else 0), ctypes.byref(it),
+                                ctypes.byref(res))
+        return it.value, res.value
+
+    def solve(self, Uu, p):
+        asm = self.assembler
+        assemble_vector_and_stiffness_(asm, Uu, p)
+        R = asm._residual()
+        self._x.zero_()
+        self.last_iters, self.last_resnorm = self.cg(asm.stiffness_storage, R, self._x)
+        # dU = -K^-1 R ; Uu += dU
+        self.dUu.zero_()
+        asm.ctx.call("fecx_vector_axpy", ctypes.c_int64(self.dUu.numel()), ctypes.c_double(-1.0), _ptr(self._x),
+                     _ptr(self.dUu))
+        asm.ctx.call("fecx_vector_axpy", ctypes.c_int64(Uu.numel()), ctypes.c_double(1.0), _ptr(self.dUu), _ptr(Uu))
+
+
+@dataclass
+class NewtonSolverSettings:
+    max_iters: int = 10
+    abs_increment_tol: float = 1e-12
+    abs_residual_tol: float = 1e-12
+    rel_residual_tol: float = 1e-12
+    log_callback: object = None   # (iter, norm_dU, norm_R, rel_R, converged) -> None
+
+
+@dataclass
+class NewtonRecord:
+    iteration: int
+    norm_dU: float
+    norm_R: float
+    rel_R: float
+    converged: bool
+    cg_iters: int
+
+
+class NewtonSolver:
+    def __init__(self, linear_solver, settings=None):
+        self.linear_solver = linear_solver
+        self.settings = settings or NewtonSolverSettings()
+        self.history = []
+
+    def _norm(self, v):
+        out = ctypes.c_double(0.0)
+        self.linear_solver.assembler.ctx.call("fecx_vector_norm", _ptr(v), ctypes.c_int64(v.numel()),
+                                              ctypes.byref(out))
+        return out.value
+
+    def solve(self, Uu, p):
+        s = self.settings
+        self.history = []
+        initial = 0.0
+        ls = self.linear_solver
+        for n in range(1, s.max_iters + 1):
+            ls.solve(Uu, p)
+            norm_dU = self._norm(ls.dUu)
+            norm_R = self._norm(ls.assembler.residual_unknowns)
+            if n == 1:
+                initial = norm_R
+            rel_R = norm_R / initial if initial > 0.0 else norm_R
+            converged = norm_dU < s.abs_increment_tol or norm_R < s.abs_residual_tol or rel_R < s.rel_residual_tol
+            self.history.append(NewtonRecord(n, norm_dU, norm_R, rel_R, converged, ls.last_iters))
+            if s.log_callback is not None:
+                s.log_callback(n, norm_dU, norm_R, rel_R, converged)
+            if converged:
+                break
+        return self.history
+
+
+def j2_load_steps(fec, n, steps=20, seed=0, surrogate=False, newton_settings=None, linear_settings=None,
+                  device=0, log=None, setup=None):
+    """The J2 workload as a quasi-static problem: n^3 hex8 elements, all boundary nodes driven by the macroscopic
+    strain path (E(t) x plus the smooth perturbation, as the workload's displacement), `steps` load steps, each solved
+    by Newton + CG from the previous step's solution, state committed after each step. Returns the assembler, the
+    parameters, the unknowns and the per-step Newton histories. setup(asm), if given, runs before the first step (it
+    attaches the trained network of a surrogate run)."""
+    from . import workloads
+    wl = workloads.j2_hex8(n, seed=seed)
+    path = wl.extra["path"]
+    V = fec.FunctionSpace(wl.mesh, "H1Field", "Lagrange")
+    var = fec.VectorFunction(V, "displ")
+    asm = fec.SparseMatrixAssembler(var, sparse_matrix_type="csr", device=device)
+    nodes = fec.boundary_nodes(wl.mesh)
+    wl.mesh.nodeset_nodes["_boundary"] = nodes
+    state = {"k": 1}
+
+    def bc(d):
+        def f(X, t):
+            k = state["k"]
+            u = workloads.j2_displacement(X, path[k - 1], k / len(path), seed).reshape(-1, 3)
+            return u[:, d]
+        return f
+
+    bcs = [fec.DirichletBC(name, bc(d), nset_name="_boundary") for d, name in enumerate(var.names)]
+    physics = fec.J2Surrogate() if surrogate else fec.J2Plasticity()
+    p = fec.create_parameters(wl.mesh, asm, physics, dirichlet_bcs=bcs)
+    if setup is not None:
+        setup(asm)
+    Uu = create_unknowns(asm)
+    newton = NewtonSolver(IterativeLinearSolver(asm, "cg", linear_settings), newton_settings)
+    histories = []
+    for k in range(1, steps + 1):
+        state["k"] = k
+        p.t = k / steps
+        p.update_bc_values()
+        hist = newton.solve(Uu, p)
+        asm.ctx.call("fecx_state_commit")
+        histories.append(hist)
+        if log:
+            log(k, hist)
+    return asm, p, Uu, histories, newton

@@ -165,6 +165,26 @@ int fecx_update_field_dirichlet(fecx_ctx* ctx, const double* vals, double* U);
 /* Uu = U[unknown_dofs]: extract_field_unknowns! (src/DofManagers.jl:203-213). */
 int fecx_extract_field_unknowns(fecx_ctx* ctx, const double* U, double* Uu);
 
+/* ---------------------------------------------------------------- linear solves on the assembled Jacobian
+ * The reference solves every Newton step with Krylov.jl's CG on stiffness(asm) on the CPU
+ * (IterativeLinearSolver, src/Solvers.jl:97-153). These run on the GPU on the CSR values the
+ * assembly wrote (device pointers; vectors over the unknowns, FP64). Reductions are deterministic. */
+
+/* y = K x. */
+int fecx_csr_spmv(fecx_ctx* ctx, const double* vals, const double* x, double* y);
+/* dinv = 1 / diag(K) (Jacobi preconditioner; AssembledDiagonal, src/assemblers/Diagonal.jl). */
+int fecx_csr_diagonal_inverse(fecx_ctx* ctx, const double* vals, double* dinv);
+/* Conjugate gradients for K x = b from the initial x: stops when ||r|| <= atol + rtol ||b|| (Krylov.jl's
+ * cg rule) or after maxit iterations; precond 1 = Jacobi, 0 = none. *iters and *resnorm report the
+ * iterations taken and the final ||r|| (of the recurrence). */
+int fecx_cg_solve(fecx_ctx* ctx, const double* vals, const double* b, double* x, double rtol, double atol, int maxit,
+                  int precond, int* iters, double* resnorm);
+
+/* ||v||_2 (host *out; deterministic two-stage sum): the Newton loop's norms (src/Solvers.jl:200-203). */
+int fecx_vector_norm(fecx_ctx* ctx, const double* v, int64_t n, double* out);
+/* y += a x: the Newton update map!((x, y) -> x + y, Uu, Uu, ΔUu) (src/Solvers.jl:147-149). */
+int fecx_vector_axpy(fecx_ctx* ctx, int64_t n, double a, const double* x, double* y);
+
 /* ---------------------------------------------------------------- learned constitutive surrogate
  * No counterpart in the reference, whose materials are energy functions differentiated by AD inside
  * the element loop (test/mechanics/TestMechanicsCommon.jl:140-222). A small MLP with GELU hidden
