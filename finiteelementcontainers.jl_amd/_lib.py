@@ -74,6 +74,8 @@ _SIGS = {
     "fecx_surrogate_train": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                              ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p],
     "fecx_surrogate_set_tolerance": [ctypes.c_void_p, ctypes.c_float],
+    "fecx_surrogate_freeze": [ctypes.c_void_p, ctypes.c_int],
+    "fecx_surrogate_enable": [ctypes.c_void_p, ctypes.c_int],
     "fecx_surrogate_fallbacks": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
     "fecx_material_update": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p],
     "fecx_qp_gradients": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],

@@ -74,6 +74,11 @@ T* upload(fecx_ctx* ctx, const std::string& name, const std::vector<T>& v) {
   return p;
 }
 
+inline size_t get_bytes(fecx_ctx* ctx, const std::string& name) {
+  auto it = ctx->buf.find(name);
+  return it == ctx->buf.end() ? 0 : it->second.bytes;
+}
+
 template <class T>
 T* get(fecx_ctx* ctx, const std::string& name) {
   auto it = ctx->buf.find(name);

@@ -117,7 +117,9 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   int* flag = get<int>(ctx, "flag");
   zero_int_kernel<<<1, 1, 0, ctx->stream>>>(flag);
   fecx::ElementArgs ea = element_args(ctx, U, V);
-  const bool surrogate = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
+  const bool surrogate_phys = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
+  const bool surrogate = surrogate_phys && ctx->sg_enabled;
+  if (surrogate_phys && !surrogate) ea.mp.phys = ctx->mp.phys == FECX_J2_SURROGATE ? FECX_J2 : FECX_NEOHOOKEAN;
   if (ctx->mp.phys == FECX_J2 || ctx->mp.phys == FECX_J2_SURROGATE)
     require(ea.st_old != nullptr, FECX_ERR_STATE, "state not allocated");
   if (surrogate) {

@@ -179,6 +179,9 @@ struct fecx_ctx {
   float sg_tol = 0.02f;
   int64_t sg_step = 0;                          // Adam steps taken
   int64_t sg_fallback = 0, sg_points = 0;       // last surrogate pass
+  bool sg_enabled = true;                       // off: surrogate physics run the plain model in the element kernel
+  bool sg_freeze = false;                       // on: reuse the last pass's accept/fallback decisions
+  bool sg_frozen_valid = false;
   // pipelined assembly: element chunks on stream2, reduction chunks on the context's stream
   int chunks = 0;  // 0: chosen from the element storage size
   hipStream_t stream2 = nullptr;

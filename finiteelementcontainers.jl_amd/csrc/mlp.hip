@@ -477,8 +477,16 @@ __global__ __launch_bounds__(256) void material_kernel(MatBatchArgs a) {
 #pragma unroll
     for (int o = 0; o < FT::OUT_W; ++o) y[o] = yn[o] * a.m.out_scale[o] + a.m.out_shift[o];
     if (valid) {
-      if constexpr (J2) ok = j2_surrogate(a.mp, H, so, y, a.tol, P, sn, C);
-      else ok = nh_surrogate(a.mp, H, y, a.tol, P, C);
+      // frozen decisions: a point that passed keeps the network's output (checked with an infinite tolerance, which
+      // only assembles P and the tangent data), a point that fell back stays on the plain model
+      const bool frozen = a.freeze && a.accept;
+      const bool use = frozen ? a.accept[i] != 0 : true;
+      const float tol = frozen ? 3.0e38f : a.tol;
+      if (use) {
+        if constexpr (J2) ok = j2_surrogate(a.mp, H, so, y, tol, P, sn, C);
+        else ok = nh_surrogate(a.mp, H, y, tol, P, C);
+      }
+      if (a.accept && !frozen && h == 0) a.accept[i] = ok ? 1 : 0;
     }
     const uint64_t fb = __ballot(valid && h == 0 && !ok);
     if (lane == 0 && fb && a.nfallback) atomicAdd(a.nfallback, (unsigned long long)__popcll(fb));

@@ -146,6 +146,13 @@ void surrogate_qp_pass(fecx_ctx* ctx, const float* hq, float* qpmat, hipStream_t
   a.qpmat = qpmat;
   a.flag = get<int>(ctx, "flag");
   a.mp.phys = ctx->mp.phys == FECX_J2_SURROGATE ? FECX_J2 : FECX_NEOHOOKEAN;
+  a.accept = (uint8_t*)fecx_api::get<uint8_t>(ctx, "sg_accept");
+  if (!a.accept || fecx_api::get_bytes(ctx, "sg_accept") < (size_t)a.n) {
+    a.accept = (uint8_t*)dev(ctx, "sg_accept", (size_t)a.n);
+    ctx->sg_frozen_valid = false;
+  }
+  a.freeze = ctx->sg_freeze && ctx->sg_frozen_valid ? 1 : 0;
+  ctx->sg_frozen_valid = true;
   HIPCHK(hipMemsetAsync(a.nfallback, 0, sizeof(unsigned long long), s));
   mlp_launch_material(ctx->sg_arch, true, a, s);
   ctx->sg_points = a.n;
@@ -335,6 +342,14 @@ int fecx_surrogate_set_tolerance(fecx_ctx* ctx, float tol) {
     require(tol > 0.0f, FECX_ERR_ARG, "tolerance must be positive");
     ctx->sg_tol = tol;
   });
+}
+
+int fecx_surrogate_freeze(fecx_ctx* ctx, int on) {
+  return guard(ctx, [&] { ctx->sg_freeze = on != 0; });
+}
+
+int fecx_surrogate_enable(fecx_ctx* ctx, int on) {
+  return guard(ctx, [&] { ctx->sg_enabled = on != 0; });
 }
 
 int fecx_surrogate_fallbacks(fecx_ctx* ctx, int64_t* n_fallback, int64_t* n_points) {

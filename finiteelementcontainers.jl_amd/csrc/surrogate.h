@@ -62,6 +62,8 @@ struct MatBatchArgs {
   float tol;
   unsigned long long* nfallback;
   int* flag;
+  uint8_t* accept;        // per point: 1 = network output used, 0 = plain model (assembly mode)
+  int freeze;             // 1: reuse `accept` from the previous pass instead of checking again
 };
 
 void mlp_launch_train(int arch, const TrainArgs& a, hipStream_t s);

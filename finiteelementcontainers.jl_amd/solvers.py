@@ -90,12 +90,15 @@ class NewtonSolver:
                                               ctypes.byref(out))
         return out.value
 
-    def solve(self, Uu, p):
+    def solve(self, Uu, p, before_iteration=None):
+        """before_iteration(n, history), if given, runs before each linear solve (surrogate schedules use it)."""
         s = self.settings
         self.history = []
         initial = 0.0
         ls = self.linear_solver
         for n in range(1, s.max_iters + 1):
+            if before_iteration is not None:
+                before_iteration(n, self.history)
             ls.solve(Uu, p)
             norm_dU = self._norm(ls.dUu)
             norm_R = self._norm(ls.assembler.residual_unknowns)
@@ -111,13 +114,48 @@ class NewtonSolver:
         return self.history
 
 
+class SurrogateSchedule:
+    """Newton with a surrogate material: iteration 1 checks every quadrature point (network or plain fallback), later
+    iterations keep those decisions (a fixed residual function, so Newton can converge on it), and once the surrogate
+    iterations stall (residual ratio above `stall`) or reach `handover`, the remaining iterations use the plain model,
+    which carries the solve to the plain method's tolerance. Records per step how many iterations each phase took."""
+
+    def __init__(self, net, handover=1e-3, stall=0.5):
+        self.net, self.handover, self.stall = net, handover, stall
+        self.phase = []       # per iteration of the current step: "surrogate" or "plain"
+        self.fallbacks = []   # per surrogate iteration: quadrature points on the plain model
+
+    def __call__(self, n, history):
+        if n > 1 and self.phase[-1] == "surrogate":
+            self.fallbacks.append(self.net.fallbacks()[0])
+        if n == 1:
+            self.phase, self.fallbacks = ["surrogate"], []
+            self.net.enable(True)
+            self.net.freeze(False)
+            return
+        if n == 2:
+            self.net.freeze(True)
+        if self.phase[-1] == "surrogate":
+            r = history[-1].rel_R
+            prev = history[-2].rel_R if len(history) >= 2 else 0.0
+            if r < self.handover or (prev > 0.0 and r / prev > self.stall):
+                self.net.enable(False)
+                self.phase.append("plain")
+            else:
+                self.phase.append("surrogate")
+        else:
+            self.phase.append("plain")
+
+
 def j2_load_steps(fec, n, steps=20, seed=0, surrogate=False, newton_settings=None, linear_settings=None,
-                  device=0, log=None, setup=None):
+                  device=0, log=None, setup=None, schedule=None):
     """The J2 workload as a quasi-static problem: n^3 hex8 elements, all boundary nodes driven by the macroscopic
     strain path (E(t) x plus the smooth perturbation, as the workload's displacement), `steps` load steps, each solved
-    by Newton + CG from the previous step's solution, state committed after each step. Returns the assembler, the
+    by Newton + CG from the previous step's solution plus the increment of the prescribed field (extended to the
+    interior), state committed after each step. Returns the assembler, the
     parameters, the unknowns and the per-step Newton histories. setup(asm), if given, runs before the first step (it
-    attaches the trained network of a surrogate run)."""
+    attaches the trained network of a surrogate run); schedule, if given, is the Newton iteration hook
+    (SurrogateSchedule)."""
     from . import workloads
     wl = workloads.j2_hex8(n, seed=seed)
     path = wl.extra["path"]
@@ -143,11 +181,17 @@ def j2_load_steps(fec, n, steps=20, seed=0, surrogate=False, newton_settings=Non
     Uu = create_unknowns(asm)
     newton = NewtonSolver(IterativeLinearSolver(asm, "cg", linear_settings), newton_settings)
     histories = []
+    unknown = asm.dof.unknown_dofs - 1
+    prev = np.zeros(len(unknown))
     for k in range(1, steps + 1):
         state["k"] = k
         p.t = k / steps
         p.update_bc_values()
-        hist = newton.solve(Uu, p)
+        # predictor: the increment of the prescribed field, extended to the interior unknowns
+        cur = workloads.j2_displacement(wl.mesh.coords, path[k - 1], k / len(path), seed)[unknown]
+        Uu += torch.from_numpy(cur - prev).to(Uu.device)
+        prev = cur
+        hist = newton.solve(Uu, p, schedule)
         asm.ctx.call("fecx_state_commit")
         histories.append(hist)
         if log:

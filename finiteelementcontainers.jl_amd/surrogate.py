@@ -98,6 +98,14 @@ class Surrogate:
     def set_tolerance(self, tol):
         self.ctx.call("fecx_surrogate_set_tolerance", ctypes.c_float(tol))
 
+    def freeze(self, on=True):
+        """Reuse the last pass's per-point network/plain decisions (a fixed residual function during a solve)."""
+        self.ctx.call("fecx_surrogate_freeze", ctypes.c_int(1 if on else 0))
+
+    def enable(self, on=True):
+        """Off: assemblies with the surrogate physics use the plain model."""
+        self.ctx.call("fecx_surrogate_enable", ctypes.c_int(1 if on else 0))
+
     def fallbacks(self):
         nf, npts = ctypes.c_int64(0), ctypes.c_int64(0)
         self.ctx.call("fecx_surrogate_fallbacks", ctypes.byref(nf), ctypes.byref(npts))

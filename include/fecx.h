@@ -227,6 +227,13 @@ int fecx_surrogate_train(fecx_ctx* ctx, const float* X, const float* T, int64_t 
                          uint64_t seed, double* loss_per_epoch);
 /* Residual-check tolerance of the surrogate material update (relative; default 0.02). */
 int fecx_surrogate_set_tolerance(fecx_ctx* ctx, float tol);
+/* on = 1: the next surrogate passes reuse the last pass's per-point decisions (network output or plain
+ * model) instead of checking again, so that the residual is one fixed function of the field during a
+ * Newton solve; on = 0 (default): every pass checks. */
+int fecx_surrogate_freeze(fecx_ctx* ctx, int on);
+/* on = 0: assemblies with a surrogate physics use the plain model (state and pattern untouched);
+ * on = 1 (default): the surrogate runs. */
+int fecx_surrogate_enable(fecx_ctx* ctx, int on);
 /* Number of quadrature points that fell back to the plain model in the last surrogate pass. */
 int fecx_surrogate_fallbacks(fecx_ctx* ctx, int64_t* n_fallback, int64_t* n_points);
 

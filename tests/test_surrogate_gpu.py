@@ -154,3 +154,42 @@ def test_surrogate_assembly_all_fallback_equals_plain(fec, gpu):
                 assert nfb == npts == wl.mesh.num_elements * 8
         for a, b in zip(out[0], out[1]):
             assert np.max(np.abs(a - b)) <= 1e-6 * np.max(np.abs(a)), np.max(np.abs(a - b)) / np.max(np.abs(a))
+
+
+def test_newton_with_surrogate_reaches_plain_tolerance(fec, gpu):
+    """J2 load steps solved by Newton + CG twice: with the plain model, and with the trained surrogate under the
+    schedule (decide, freeze, hand over to the plain model when the surrogate iterations stall). Both reach the same
+    relative residual tolerance on every step and the same solution."""
+    print("[surrogate] Newton plain vs surrogate", flush=True)
+    _, asm0, _ = _asm(fec, gpu, fec.J2Surrogate(*J2_PROPS))
+    X, T = fec.surrogate.j2_generate_samples(asm0, 4000, steps=20, max_norm=0.01, seed=1)
+    net0 = fec.Surrogate(asm0, fec.surrogate.MLP_J2, seed=0)
+    net0.fit_scaling(X, T)
+    net0.train(X, T, epochs=12, batch=2048, lr=2e-3, seed=0)
+    params, scaling = net0.params(), net0.scaling()
+    st = fec.NewtonSolverSettings(max_iters=15, rel_residual_tol=1e-5, abs_residual_tol=0.0, abs_increment_tol=0.0)
+    out = {}
+    for surr in (False, True):
+        holder = {}
+
+        def setup(asm):
+            if surr:
+                net = fec.Surrogate(asm, fec.surrogate.MLP_J2, seed=0)
+                net.set_params(params)
+                net.set_scaling(*scaling)
+                net.set_tolerance(0.05)
+                holder["s"] = fec.solvers.SurrogateSchedule(net)
+
+        def schedule(n, h):
+            if "s" in holder:
+                holder["s"](n, h)
+
+        asm, p, Uu, hists, _ = fec.solvers.j2_load_steps(fec, 6, steps=6, newton_settings=st, setup=setup,
+                                                          surrogate=surr, schedule=schedule)
+        out[surr] = (Uu.cpu().numpy().copy(), hists)
+        print(f"[surrogate] {'surrogate' if surr else 'plain'}: iterations per step "
+              f"{[len(h) for h in hists]}, final rel {[f'{h[-1].rel_R:.1e}' for h in hists]}", flush=True)
+        assert all(h[-1].converged for h in hists)
+    d = np.linalg.norm(out[True][0] - out[False][0]) / np.linalg.norm(out[False][0])
+    print(f"[surrogate] solution difference {d:.2e}", flush=True)
+    assert d < 1e-5
