@@ -82,6 +82,91 @@ def pmc_traffic(workload, kernel):
     return None
 
 
+def cuda_ms(torch, f, reps=10):
+    """Average milliseconds of f() between HIP events on the current stream (the assembler's stream)."""
+    f()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        f()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+MLP_FLOPS = {"j2_hex8": 2 * (13 * 128 + 128 * 128 * 2 + 128 * 13), "neohookean_tet10": 2 * (9 * 256 + 256 * 256 + 256 * 15)}
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_leg(fec, torch, name, wl, asm_plain, R_plain, K_plain, steps):
+    """The learned path of the J2 and tet10 neo-Hookean configurations: train the network on the GPU on plain-model
+    samples (J2: 2M samples of random macroscopic strain paths; NH: 2M quadrature points of this workload), then
+    measure the held-out acceptance and error at the residual-check tolerance 0.05, the batched material update of
+    2M points (plain vs surrogate with check and fallback vs the network alone, HIP events), and one residual +
+    Jacobian assembly of the whole workload with the surrogate physics against the plain one."""
+    import copy
+    S, L = fec.surrogate, fec._lib
+    j2 = name == "j2_hex8"
+    plain_kind, surr_kind = (L.J2, L.J2_SURROGATE) if j2 else (L.NEOHOOKEAN, L.NH_SURROGATE)
+    wl_s = copy.copy(wl)
+    wl_s.physics = fec.J2Surrogate(*wl.physics.p) if j2 else fec.NeoHookeanSurrogate(*wl.physics.p)
+    asm, p = build_assembler(fec, wl_s)
+    t0 = time.time()
+    if j2:
+        X, T = S.j2_generate_samples(asm, 100000, steps=20, max_norm=0.01, seed=1)
+        Xv, _ = S.j2_generate_samples(asm, 5000, steps=20, max_norm=0.01, seed=2)
+    else:
+        H = S.qp_gradients(asm, torch.from_numpy(wl.U).to(asm.device)).reshape(-1, 9)
+        g = torch.Generator(device=asm.device)
+        g.manual_seed(0)
+        perm = torch.randperm(H.shape[0], device=asm.device, generator=g)
+        X, Xv = H[perm[:2_000_000]].contiguous(), H[perm[2_000_000:2_200_000]].contiguous()
+        T = S.material_update(asm, plain_kind, X)
+        del H, perm
+    net = fec.Surrogate(asm, S.MLP_J2 if j2 else S.MLP_NH, seed=0)
+    net.fit_scaling(X, T)
+    losses = []
+    for lr, ep in ((2e-3, 15), (1e-3, 15), (3e-4, 15), (1e-4, 10)):
+        losses.append(net.train(X, T, epochs=ep, batch=16384, lr=lr, seed=int(lr * 1e6)))
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    net.set_tolerance(0.05)
+    Yv = S.material_update(asm, surr_kind, Xv)
+    fb, nv = net.fallbacks()
+    Pv = S.material_update(asm, plain_kind, Xv)
+    ns = 6 if j2 else 9
+    stress_err = ((Yv[:, :ns] - Pv[:, :ns]).abs().max() / Pv[:, :ns].abs().max()).item()
+    nb = X.shape[0]
+    t_plain = cuda_ms(torch, lambda: S.material_update(asm, plain_kind, X))
+    t_surr = cuda_ms(torch, lambda: S.material_update(asm, surr_kind, X))
+    fb_b = net.fallbacks()[0]
+    t_net = cuda_ms(torch, lambda: net.forward(X))
+    tflops = MLP_FLOPS[name] * nb / (t_net * 1e-3) / 1e12
+    log(f"[bench] surrogate: trained {t_train:.1f}s, held-out fallback {fb / nv:.3f}, batched update 2M points "
+        f"plain {t_plain:.3f} ms, surrogate {t_surr:.3f} ms, network {t_net:.3f} ms ({tflops:.1f} TFLOP/s)")
+    # the whole workload with the surrogate physics
+    Uu = torch.from_numpy(wl.U[asm.dof.unknown_dofs - 1]).to(asm.device)
+    ms = cuda_ms(torch, lambda: fec.assemble_vector_and_stiffness_(asm, Uu, p), reps=max(3, min(steps, 10)))
+    fb_a, np_a = net.fallbacks()
+    R = fec.residual(asm).cpu().numpy()
+    K = asm.stiffness_storage.cpu().numpy()
+    out = {
+        "network": "13-128-128-128-13" if j2 else "9-256-256-15", "activation": "GELU", "arithmetic": "bf16 MFMA, fp32 accumulation",
+        "train_samples": int(nb), "train_s": t_train, "epochs": int(sum(len(x) for x in losses)),
+        "final_loss": float(losses[-1][-1]), "check_tolerance": 0.05,
+        "heldout": {"samples": int(nv), "fallback_fraction": fb / nv, "max_stress_error_rel": stress_err},
+        "batched_update_ms_2M": {"plain": t_plain, "surrogate": t_surr, "network_only": t_net,
+                                 "surrogate_fallback_fraction": fb_b / nb},
+        "throughput_ratio_vs_plain": t_plain / t_surr,
+        "mfma": {"achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS},
+        "assembly_ms_per_step": {"plain_step_time_above": None, "surrogate": ms},
+        "assembly_fallback_fraction": fb_a / max(np_a, 1),
+        "assembly_rel_diff_vs_plain": {"residual": rel(R, R_plain), "jacobian": rel(K, K_plain)},
+    }
+    return out
+
+
 def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
@@ -130,6 +215,7 @@ def main():
     ap.add_argument("--chunks", type=int, default=0, help="pipelined pieces per matrix assembly (0 or 1: one piece)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the sample parity/CPU legs (profiling runs)")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg (J2, tet10 NH)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -214,6 +300,12 @@ def main():
     if name in SURVEY_BYTES:
         eff = SURVEY_BYTES[name] * wl.num_elements / dt / 1e9
         out["step_roofline"] = {"bytes_per_element": SURVEY_BYTES[name], "achieved": eff, "frac": eff / HBM_PEAK_GBS}
+
+    if name in MLP_FLOPS and not args.no_surrogate:
+        R_plain = fec.residual(asm).cpu().numpy().copy()
+        K_plain = asm.stiffness_storage.cpu().numpy().copy()
+        out["surrogate"] = surrogate_leg(fec, torch, name, wl, asm, R_plain, K_plain, args.steps)
+        out["surrogate"]["assembly_ms_per_step"]["plain_step_time_above"] = dt * 1e3
 
     if not args.no_check:
         orc = ge.load_oracle()

@@ -147,53 +147,95 @@ class SurrogateSchedule:
             self.phase.append("plain")
 
 
-def j2_load_steps(fec, n, steps=20, seed=0, surrogate=False, newton_settings=None, linear_settings=None,
-                  device=0, log=None, setup=None, schedule=None):
+class J2LoadPath:
     """The J2 workload as a quasi-static problem: n^3 hex8 elements, all boundary nodes driven by the macroscopic
-    strain path (E(t) x plus the smooth perturbation, as the workload's displacement), `steps` load steps, each solved
-    by Newton + CG from the previous step's solution plus the increment of the prescribed field (extended to the
-    interior), state committed after each step. Returns the assembler, the
-    parameters, the unknowns and the per-step Newton histories. setup(asm), if given, runs before the first step (it
-    attaches the trained network of a surrogate run); schedule, if given, is the Newton iteration hook
-    (SurrogateSchedule)."""
-    from . import workloads
-    wl = workloads.j2_hex8(n, seed=seed)
-    path = wl.extra["path"]
-    V = fec.FunctionSpace(wl.mesh, "H1Field", "Lagrange")
-    var = fec.VectorFunction(V, "displ")
-    asm = fec.SparseMatrixAssembler(var, sparse_matrix_type="csr", device=device)
-    nodes = fec.boundary_nodes(wl.mesh)
-    wl.mesh.nodeset_nodes["_boundary"] = nodes
-    state = {"k": 1}
+    strain path (E(t) x plus the smooth perturbation, as the workload's displacement), each load step solved by
+    Newton + CG from the previous step's solution plus the increment of the prescribed field (extended to the
+    interior), state committed after each step."""
 
-    def bc(d):
+    def __init__(self, fec, n, seed=0, surrogate=False, newton_settings=None, linear_settings=None, device=0):
+        from . import workloads
+        self.fec, self.seed, self.workloads = fec, seed, workloads
+        self.wl = wl = workloads.j2_hex8(n, seed=seed)
+        self.path = wl.extra["path"]
+        V = fec.FunctionSpace(wl.mesh, "H1Field", "Lagrange")
+        var = fec.VectorFunction(V, "displ")
+        self.asm = asm = fec.SparseMatrixAssembler(var, sparse_matrix_type="csr", device=device)
+        wl.mesh.nodeset_nodes["_boundary"] = fec.boundary_nodes(wl.mesh)
+        self.k = 1
+        bcs = [fec.DirichletBC(name, self._bc(d), nset_name="_boundary") for d, name in enumerate(var.names)]
+        physics = fec.J2Surrogate() if surrogate else fec.J2Plasticity()
+        self.p = fec.create_parameters(wl.mesh, asm, physics, dirichlet_bcs=bcs)
+        self.Uu = create_unknowns(asm)
+        self.newton = NewtonSolver(IterativeLinearSolver(asm, "cg", linear_settings), newton_settings)
+        self._unknown = asm.dof.unknown_dofs - 1
+        self._prev = np.zeros(len(self._unknown))
+
+    def _field(self, k):
+        return self.workloads.j2_displacement(self.wl.mesh.coords, self.path[k - 1], k / len(self.path), self.seed)
+
+    def _bc(self, d):
         def f(X, t):
-            k = state["k"]
-            u = workloads.j2_displacement(X, path[k - 1], k / len(path), seed).reshape(-1, 3)
-            return u[:, d]
+            u = self.workloads.j2_displacement(X, self.path[self.k - 1], self.k / len(self.path), self.seed)
+            return u.reshape(-1, 3)[:, d]
         return f
 
-    bcs = [fec.DirichletBC(name, bc(d), nset_name="_boundary") for d, name in enumerate(var.names)]
-    physics = fec.J2Surrogate() if surrogate else fec.J2Plasticity()
-    p = fec.create_parameters(wl.mesh, asm, physics, dirichlet_bcs=bcs)
+    def step(self, k, schedule=None):
+        """Load step k (1-based): boundary values, predictor, Newton; then the state commit."""
+        self.k = k
+        self.p.t = k / len(self.path)
+        self.p.update_bc_values()
+        cur = self._field(k)[self._unknown]
+        self.Uu += torch.from_numpy(cur - self._prev).to(self.Uu.device)
+        self._prev = cur
+        hist = self.newton.solve(self.Uu, self.p, schedule)
+        self.asm.ctx.call("fecx_state_commit")
+        return hist
+
+    def state(self):
+        """The committed quadrature state (ne, 8, 7) FP32 on the host."""
+        ne = self.wl.mesh.num_elements
+        st = np.zeros(ne * 8 * 7, dtype=np.float32)
+        self.asm.ctx.call("fecx_get_state_new", st.ctypes.data_as(ctypes.c_void_p))
+        return st.reshape(ne, 8, 7)
+
+
+def j2_load_steps(fec, n, steps=20, seed=0, surrogate=False, newton_settings=None, linear_settings=None,
+                  device=0, log=None, setup=None, schedule=None):
+    """Runs `steps` load steps of J2LoadPath. setup(asm), if given, runs before the first step (it attaches the
+    trained network of a surrogate run); schedule, if given, is the Newton iteration hook (SurrogateSchedule).
+    Returns the assembler, the parameters, the unknowns, the per-step Newton histories and the Newton solver."""
+    lp = J2LoadPath(fec, n, seed, surrogate, newton_settings, linear_settings, device)
     if setup is not None:
-        setup(asm)
-    Uu = create_unknowns(asm)
-    newton = NewtonSolver(IterativeLinearSolver(asm, "cg", linear_settings), newton_settings)
+        setup(lp.asm)
     histories = []
-    unknown = asm.dof.unknown_dofs - 1
-    prev = np.zeros(len(unknown))
     for k in range(1, steps + 1):
-        state["k"] = k
-        p.t = k / steps
-        p.update_bc_values()
-        # predictor: the increment of the prescribed field, extended to the interior unknowns
-        cur = workloads.j2_displacement(wl.mesh.coords, path[k - 1], k / len(path), seed)[unknown]
-        Uu += torch.from_numpy(cur - prev).to(Uu.device)
-        prev = cur
-        hist = newton.solve(Uu, p, schedule)
-        asm.ctx.call("fecx_state_commit")
+        hist = lp.step(k, schedule)
         histories.append(hist)
         if log:
             log(k, hist)
-    return asm, p, Uu, histories, newton
+    return lp.asm, lp.p, lp.Uu, histories, lp.newton
+
+
+def j2_collect_samples(fec, n, steps=20, seed=0, newton_settings=None, device=0):
+    """Training samples of the plain J2 model on the workload itself: the load steps are solved with the plain model,
+    and before every Newton iteration every quadrature point contributes its raw input (displacement gradient at the
+    current iterate, state at the start of the step) labelled by the plain update. Returns X (m, 16) and T (m, 13) on
+    the device, m = ne x 8 x (Newton iterations)."""
+    from . import surrogate as S
+    from .assembler import _update_for_assembly
+    lp = J2LoadPath(fec, n, seed, False, newton_settings, None, device)
+    state = torch.zeros((lp.wl.mesh.num_elements, 8, 7), dtype=torch.float32, device=lp.asm.device)
+    xs = []
+
+    def before(k_iter, history):
+        _update_for_assembly(lp.p, lp.asm, lp.Uu)
+        H = S.qp_gradients(lp.asm, lp.p.field)
+        xs.append(torch.cat([H, state], dim=2).reshape(-1, 16))
+
+    for k in range(1, steps + 1):
+        lp.step(k, before)
+        state.copy_(torch.from_numpy(lp.state()))
+    X = torch.cat(xs, dim=0).contiguous()
+    T = S.material_update(lp.asm, fec._lib.J2, X)
+    return X, T
