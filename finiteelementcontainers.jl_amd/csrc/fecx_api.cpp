@@ -84,6 +84,8 @@ fecx::ReduceArgs reduce_args(fecx_ctx* ctx) {
   r.nnpe = ctx->nnpe;
   r.vstride = ctx->hp.vstride;
   r.maxdeg = ctx->hp.maxdeg;
+  r.nlight = ctx->hp.nlight;
+  r.maxdeg_light = ctx->hp.maxdeg_light;
   r.free_rank = get<int32_t>(ctx, "free_rank");
   r.bptr = get<int64_t>(ctx, "bptr");
   r.voff = get<int32_t>(ctx, "voff");
@@ -100,6 +102,7 @@ fecx::ReduceArgs reduce_args(fecx_ctx* ctx) {
 // kernels competing for the same CU resources.
 int pipeline_chunks(fecx_ctx* ctx, int mode) {
   if (!(mode & fecx::MODE_K)) return 1;
+  if (ctx->hp.nlight < ctx->nn) return 1;  // two row buckets: the chunk keys assume one Morton-ordered bucket
   return ctx->chunks > 0 ? ctx->chunks : 1;
 }
 
@@ -296,7 +299,7 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
                              ctx->pairs, ctx->nd, ctx->coords.data());
     const int nf = ctx->nf;
     const auto& hp = ctx->hp;
-    require(hp.maxdeg < 255, FECX_ERR_UNSUPPORTED, "a node couples to more than 254 nodes");
+    require(hp.maxdeg <= 128, FECX_ERR_UNSUPPORTED, "a node couples to more than 128 nodes");
     ctx->nnz = hp.nnzb * nf * nf;
     std::vector<int32_t> conn_s(ctx->ne * ctx->nnpe);  // connectivity in element storage order
     for (int64_t s = 0; s < ctx->ne; ++s)

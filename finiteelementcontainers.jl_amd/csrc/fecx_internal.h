@@ -69,7 +69,9 @@ struct HostPattern {
   std::vector<int32_t> free_node;   // nfree: node id (0-based)
   std::vector<int64_t> bptr;        // nfree + 1 : block row pointer
   std::vector<int32_t> bcol;        // nnzb: free rank of the column node, ascending per row
-  std::vector<int32_t> norder;      // processing position -> node (Morton order)
+  std::vector<int32_t> norder;      // processing position -> node: light rows, then heavy rows, Morton order in each
+  int64_t nlight = 0;               // light rows: at most 8 incidences and 32 column blocks
+  int maxdeg_light = 0;
   std::vector<int32_t> eperm;       // storage position -> element
   std::vector<int32_t> epos;        // element -> storage position
   std::vector<int64_t> ekey;        // nn + 1: storage positions of the elements whose smallest node rank is < p
@@ -115,6 +117,8 @@ int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t
 struct ReduceArgs {
   int64_t nn;
   int64_t p0, p1;           // processing positions [p0, p1) of this launch
+  int64_t nlight;           // positions [0, nlight) hold the light rows (at most 8 incidences and 32 blocks)
+  int maxdeg_light;         // largest row degree among them
   const int32_t* norder;
   int nnpe, vstride, maxdeg;
   const int32_t* free_rank;

@@ -173,6 +173,29 @@ void build_host_pattern(HostPattern& hp, int64_t nn, int64_t ne, int nnpe, const
     hp.maxdeg = std::max<int>(hp.maxdeg, (int)deg[r]);
   }
   hp.nnzb = hp.bptr[nfree];
+  // processing order: light rows (at most 8 incidences, one reduction pass, and at most 32 column blocks) first, then
+  // the heavy ones, each bucket in Morton order
+  {
+    std::vector<int32_t> light, heavy;
+    light.reserve(nn);
+    int mdl = 0;
+    for (int64_t i = 0; i < nn; ++i) {
+      const int32_t n = hp.norder[i];
+      const int ninc = hp.voff[n + 1] - hp.voff[n];
+      const int32_t r = hp.free_rank[n];
+      const int d = r >= 0 ? (int)deg[r] : 0;
+      if (ninc <= 8 && d <= 32) {
+        light.push_back(n);
+        mdl = std::max(mdl, d);
+      } else {
+        heavy.push_back(n);
+      }
+    }
+    hp.nlight = (int64_t)light.size();
+    hp.maxdeg_light = mdl;
+    std::copy(light.begin(), light.end(), hp.norder.begin());
+    std::copy(heavy.begin(), heavy.end(), hp.norder.begin() + light.size());
+  }
   hp.bcol.resize(hp.nnzb);
   hp.vslot.assign((size_t)ne * nnpe * hp.vstride, 0xFF);
   // pass 2: columns and the per-incidence column slots

@@ -8,6 +8,8 @@
 // adjacent lanes writing adjacent columns.
 // Vector: the same groups sum the node's incident element rows into the residual field (the atomic scatter of
 // src/assemblers/Assemblers.jl:72-87 restated as an owner-computes gather).
+#include <algorithm>
+
 #include "fecx_internal.h"
 
 namespace fecx {
@@ -142,19 +144,24 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
 #pragma unroll
         for (int i = 0; i < NF; ++i)
 #pragma unroll
-          for (int k = 0; k < NF; ++k) a.vals[base + (int64_t)i * NF * deg + k] = acc[sw][i * NF + k];
+          for (int k = 0; k < NF; ++k) {
+            double* dst = &a.vals[base + (int64_t)i * NF * deg + k];
+            if constexpr (NF >= 3) __builtin_nontemporal_store(acc[sw][i * NF + k], dst);  // written once, never read here
+            else *dst = acc[sw][i * NF + k];
+          }
       }
     }
   }
 }
 
+// One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 16 for
+// scalar rows of up to 32 blocks (hex8: 27), 32 for up to 32 blocks, else 64 with two sweeps (tet10 vertex rows: 65).
 template <int NF, int NNPE>
-static int launch_lpr(const ReduceArgs& a, hipStream_t s) {
-  // 16 lanes x 2 sweeps cover rows of up to 32 blocks (hex8: 27), 32 x 2 up to 64, 64 x 2 up to 128
-  // NF = 1: 16 lanes x 2 sweeps cover rows of up to 32 blocks (hex8: 27); NF = 3: one sweep of 32 lanes
-  const int lpr = (NF == 1 && a.maxdeg <= 32) ? 16 : (a.maxdeg <= 32 ? 32 : (a.maxdeg <= 64 ? 32 : 64));
-  if (a.maxdeg > 2 * lpr) return -1;
-  const int sw = a.maxdeg <= lpr ? 1 : 2;
+static int launch_lpr(const ReduceArgs& a, int maxdeg, hipStream_t s) {
+  if (a.p1 <= a.p0) return 0;
+  const int lpr = (NF == 1 && maxdeg <= 32) ? 16 : (maxdeg <= 32 ? 32 : 64);
+  if (maxdeg > 2 * lpr) return -1;
+  const int sw = maxdeg <= lpr ? 1 : 2;
   const int64_t npw = RED_THREADS / lpr;
   int64_t grid = (a.p1 - a.p0 + npw - 1) / npw;
   grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
@@ -167,15 +174,28 @@ static int launch_lpr(const ReduceArgs& a, hipStream_t s) {
   return 0;
 }
 
+// The processing order puts the light rows (at most CH incidences and 32 blocks) first and the heavy ones after
+// (tet10: edge-midpoint rows, then vertex rows); each bucket gets the lane count that fits it, so light rows neither
+// idle on a heavy row's lane count nor wait at a work-group barrier through its extra passes.
+template <int NF, int NNPE>
+static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
+  ReduceArgs lo = a, hi = a;
+  lo.p1 = std::min(a.p1, std::max(a.p0, a.nlight));
+  hi.p0 = std::max(a.p0, a.nlight);
+  int rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
+  if (rc == 0) rc = launch_lpr<NF, NNPE>(hi, a.maxdeg, s);
+  return rc;
+}
+
 int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s) {
   if (a.p1 <= a.p0) return 0;
   if (nf == 1) {
-    if (a.nnpe == 4) return launch_lpr<1, 4>(a, s);
-    if (a.nnpe == 8) return launch_lpr<1, 8>(a, s);
-    if (a.nnpe == 10) return launch_lpr<1, 10>(a, s);
+    if (a.nnpe == 4) return launch_buckets<1, 4>(a, s);
+    if (a.nnpe == 8) return launch_buckets<1, 8>(a, s);
+    if (a.nnpe == 10) return launch_buckets<1, 10>(a, s);
   } else if (nf == 3) {
-    if (a.nnpe == 8) return launch_lpr<3, 8>(a, s);
-    if (a.nnpe == 10) return launch_lpr<3, 10>(a, s);
+    if (a.nnpe == 8) return launch_buckets<3, 8>(a, s);
+    if (a.nnpe == 10) return launch_buckets<3, 10>(a, s);
   }
   return -1;
 }
