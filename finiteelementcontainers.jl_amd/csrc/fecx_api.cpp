@@ -81,6 +81,7 @@ fecx::ReduceArgs reduce_args(fecx_ctx* ctx) {
   r.p0 = 0;
   r.p1 = ctx->nn;
   r.norder = get<int32_t>(ctx, "norder");
+  r.rowinfo = get<int4>(ctx, "rowinfo");
   r.nnpe = ctx->nnpe;
   r.vstride = ctx->hp.vstride;
   r.maxdeg = ctx->hp.maxdeg;
@@ -308,6 +309,22 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
     upload(ctx, "eperm", hp.eperm);
     upload(ctx, "epos", hp.epos);
     upload(ctx, "norder", hp.norder);
+    {  // reduction row descriptors in processing order (one load instead of a chain of dependent ones)
+      require(hp.nnzb < (int64_t(1) << 31), FECX_ERR_UNSUPPORTED, "more than 2^31 - 1 CSR blocks");
+      std::vector<int32_t> ri((size_t)ctx->nn * 4);
+      for (int64_t p = 0; p < ctx->nn; ++p) {
+        const int32_t n = hp.norder[p];
+        const int32_t r = hp.free_rank[n];
+        const int64_t deg = r >= 0 ? hp.bptr[r + 1] - hp.bptr[r] : 0;
+        const int32_t ninc = hp.voff[n + 1] - hp.voff[n];
+        require(ninc < 65536, FECX_ERR_UNSUPPORTED, "a node has more than 65535 incident elements");
+        ri[4 * p + 0] = n;
+        ri[4 * p + 1] = hp.voff[n];
+        ri[4 * p + 2] = r >= 0 ? (int32_t)hp.bptr[r] : 0;
+        ri[4 * p + 3] = ninc | (int32_t)(deg << 16);
+      }
+      upload(ctx, "rowinfo", ri);
+    }
     upload(ctx, "free_rank", hp.free_rank);
     upload(ctx, "bptr", hp.bptr);
     upload(ctx, "bcol", hp.bcol);

@@ -120,6 +120,7 @@ struct ReduceArgs {
   int64_t nlight;           // positions [0, nlight) hold the light rows (at most 8 incidences and 32 blocks)
   int maxdeg_light;         // largest row degree among them
   const int32_t* norder;
+  const int4* rowinfo;      // per processing position: {node, voff[node], bptr[rank] or 0, ninc | degree << 16}
   int nnpe, vstride, maxdeg;
   const int32_t* free_rank;
   const int64_t* bptr;

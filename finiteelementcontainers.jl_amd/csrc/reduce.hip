@@ -62,22 +62,20 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
   const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
   const int64_t pos = a.p0 + chunk * NPW + grp;
   const bool valid = pos < a.p1;
-  const int64_t n = valid ? a.norder[pos] : 0;
-  int32_t c0 = 0, ninc = 0;
-  if (valid) { c0 = a.voff[n]; ninc = a.voff[n + 1] - c0; }
+  // one 16-byte row descriptor: {node, first incidence, first block of the row, incidences | degree << 16}
+  const int4 ri = valid ? a.rowinfo[pos] : make_int4(0, 0, 0, 0);
+  const int64_t n = ri.x;
+  const int32_t c0 = ri.y, ninc = ri.w & 0xFFFF;
   const int npass = (ninc + CH - 1) / CH;
   if (threadIdx.x == 0) s_maxpass = 0;
   __syncthreads();
   if (lane == 0) atomicMax(&s_maxpass, npass);
   __syncthreads();
   const int maxpass = s_maxpass;
-  int32_t r = -1;
-  int64_t rb = 0;
-  int deg = 0;
-  if (valid && a.vals) {
-    r = a.free_rank[n];
-    if (r >= 0) { rb = a.bptr[r]; deg = (int)(a.bptr[r + 1] - rb); }
-  }
+  // a row of a node with Dirichlet dofs has no CSR blocks (degree 0); a free node always couples with itself
+  const int deg = a.vals ? (ri.w >> 16) : 0;
+  const int32_t r = deg > 0 ? 0 : -1;
+  const int64_t rb = ri.z;
   double acc[SW][NF * NF];
 #pragma unroll
   for (int t = 0; t < SW; ++t)
