@@ -99,9 +99,22 @@ struct MlpDev {
   const float* out_scale;
 };
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU in its tanh form, 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))) = z sigma(u), u = 2 sqrt(2/pi) (z + 0.044715 z^3)
+// (Hendrycks & Gimpel, "Gaussian Error Linear Units"), evaluated with the hardware exp2 and reciprocal: five VALU
+// operations and two transcendental ones per activation (the erf form costs several times that, and the network's
+// hidden activations dominate its VALU work). exp2 overflow gives sigma = 0, the correct limit.
+constexpr float GELU_U = 1.5957691216057308f;   // 2 sqrt(2 / pi)
+constexpr float GELU_C = 0.044715f;
+constexpr float LOG2E = 1.4426950408889634f;
+__device__ __forceinline__ float gelu_sigma(float z) {
+  const float u = GELU_U * z * fmaf(GELU_C * z, z, 1.0f);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-LOG2E * u));
+}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }
 __device__ __forceinline__ float gelu_prime(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
+  const float s = gelu_sigma(z);
+  const float du = GELU_U * fmaf(3.0f * GELU_C * z, z, 1.0f);
+  return fmaf(z * s * (1.0f - s), du, s);
 }
 
 // Z = W_l H + b for one layer; hb holds P[l] / 16 B operands, acc receives P[l+1] / 32 D tiles.

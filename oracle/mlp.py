@@ -4,7 +4,7 @@ The surrogate has no counterpart in the reference. This module is the checker of
 kernels (libfecx.so, FECX_MLP_J2 / FECX_MLP_NH), used only by tests/. It is pinned by central finite differences
 of its own loss (tests/test_surrogate_cpu.py).
 
-Network: standardized features -> [W_l h + b_l, GELU (exact, erf form)] hidden layers -> linear output. Loss: mean
+Network: standardized features -> [W_l h + b_l, GELU (tanh form)] hidden layers -> linear output. Loss: mean
 over samples and outputs of the squared error of the standardized outputs. Parameters in canonical order: per layer
 W (out x in, row-major) then b.
 
@@ -14,7 +14,6 @@ vs FP64 accumulation:
   - the output-side gradients dz_l, both where the weight gradient reads them and where the backward GEMM does.
 """
 import numpy as np
-from scipy.special import erf
 
 SHAPES = {0: (13, 128, 128, 128, 13), 1: (9, 256, 256, 15)}
 
@@ -55,12 +54,18 @@ def features(arch, raw):
     return raw[:, :9].copy()
 
 
+GELU_U = 2.0 * np.sqrt(2.0 / np.pi)
+GELU_C = 0.044715
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    """GELU, tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))) = z sigma(2 sqrt(2/pi) (z + 0.044715 z^3))."""
+    return 0.5 * z * (1.0 + np.tanh(0.5 * GELU_U * (z + GELU_C * z ** 3)))
 
 
 def gelu_prime(z):
-    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+    s = 0.5 * (1.0 + np.tanh(0.5 * GELU_U * (z + GELU_C * z ** 3)))
+    return s + z * s * (1.0 - s) * GELU_U * (1.0 + 3.0 * GELU_C * z * z)
 
 
 def forward(arch, w, scaling, raw, emulate=False):
