@@ -292,6 +292,18 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
           }
         }
       }
+      if constexpr (NF > 1) {
+        // the diagonal block K_aa of these symmetric tangents is symmetric up to the FP32 rounding of its two FMA
+        // orders; average it with its transpose so that the assembled Jacobian is symmetric bit for bit
+#pragma unroll
+        for (int i = 0; i < NF; ++i)
+#pragma unroll
+          for (int k = i + 1; k < NF; ++k) {
+            const float m = 0.5f * (acc[0][i * NF + k] + acc[0][k * NF + i]);
+            acc[0][i * NF + k] = m;
+            acc[0][k * NF + i] = m;
+          }
+      }
       if constexpr (DO_K) {
         // every lane of the work-group has finished reading the quadrature data of its element only after the
         // barrier below, so the blocks are kept in registers until then
