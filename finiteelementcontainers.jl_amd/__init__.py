@@ -5,6 +5,7 @@ The package directory name contains a dot, so load it by path (``__graft_entry__
 """
 from ._lib import FECXError, exported_symbols, LIB_PATH  # noqa: F401
 from .mesh import StructuredMesh, TetMesh, kuhn_tet10_mesh, boundary_nodes  # noqa: F401
+from .exodus import ExodusMesh, UnstructuredMesh, write_exodus  # noqa: F401
 from .assembler import (  # noqa: F401
     Poisson, Mechanics, NeoHookean, J2Plasticity, ThreeDimensional,
     residual, stiffness, stiffness_action,
@@ -14,7 +15,7 @@ from .assembler import (  # noqa: F401
     assemble_matrix_action_, assemble_matrix_free_action_, assemble_matrix_free_action_full_, hvp,
     set_chunks, set_profiling, profile, reset_profile,
 )
-from . import rng, workloads, surrogate, solvers  # noqa: F401
+from . import rng, workloads, surrogate, solvers, exodus  # noqa: F401
 from .solvers import (IterativeLinearSolver, IterativeLinearSolverSettings, NewtonSolver,  # noqa: F401
                       NewtonSolverSettings)
 from .surrogate import J2Surrogate, NeoHookeanSurrogate, Surrogate  # noqa: F401
