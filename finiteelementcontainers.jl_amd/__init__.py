@@ -8,11 +8,12 @@ from .mesh import StructuredMesh, TetMesh, kuhn_tet10_mesh, boundary_nodes  # no
 from .exodus import ExodusMesh, UnstructuredMesh, write_exodus  # noqa: F401
 from .assembler import (  # noqa: F401
     Poisson, Mechanics, NeoHookean, J2Plasticity, ThreeDimensional,
-    residual, stiffness, stiffness_action,
+    residual, stiffness, stiffness_action, mass, lumped_mass, diagonal, density,
     FunctionSpace, ScalarFunction, VectorFunction, DofManager, DirichletBC, DirichletBCs,
     CSRMatrix, SparseMatrixAssembler, HostPattern, Parameters, create_parameters, create_unknowns, create_field,
     assemble_vector_, assemble_stiffness_, assemble_matrix_, assemble_vector_and_stiffness_,
     assemble_matrix_action_, assemble_matrix_free_action_, assemble_matrix_free_action_full_, hvp,
+    assemble_mass_, assemble_lumped_mass_, assemble_diagonal_,
     set_chunks, set_profiling, profile, reset_profile,
 )
 from . import rng, workloads, surrogate, solvers, exodus  # noqa: F401

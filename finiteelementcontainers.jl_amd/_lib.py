@@ -51,6 +51,11 @@ _SIGS = {
     "fecx_assemble_vector_matrix": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_assemble_matrix_action": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_assemble_matrix_free_action": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_assemble_diagonal": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "fecx_assemble_mass": [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p],
+    "fecx_assemble_lumped_mass": [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p],
+    "fecx_assemble_mass_diagonal": [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p],
+    "fecx_assemble_mass_action": [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_update_field_unknowns": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_update_field_dirichlet": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "fecx_extract_field_unknowns": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],

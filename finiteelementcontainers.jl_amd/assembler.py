@@ -113,6 +113,35 @@ def stiffness_action(obj, *args):
     raise TypeError("stiffness_action(physics, ...) is evaluated inside libfecx.so; pass it as the callback")
 
 
+def mass(obj, *args):
+    """Callback tag of the consistent mass; mass(asm) is the assembled CSR mass matrix (Assemblers.jl:329-336)."""
+    if isinstance(obj, SparseMatrixAssembler):
+        return obj._mass()
+    raise TypeError("mass(physics, ...) is evaluated inside libfecx.so; pass `mass` as the callback")
+
+
+def lumped_mass(obj, *args):
+    """Callback tag of the row-sum lumped mass; lumped_mass(asm) is the assembled vector over the unknowns
+    (src/assemblers/LumpedMass.jl:69-80). It shares storage with residual(asm)."""
+    if isinstance(obj, SparseMatrixAssembler):
+        return obj._residual()
+    raise TypeError("lumped_mass(physics, ...) is evaluated inside libfecx.so; pass `lumped_mass` as the callback")
+
+
+def diagonal(asm):
+    """The assembled diagonal over the unknowns (src/assemblers/Diagonal.jl:70-83); shares storage with residual(asm)."""
+    return asm._residual()
+
+
+def density(physics):
+    """The density the test physics' mass callbacks multiply by: props[1] of Mechanics
+    (test/mechanics/TestMechanicsCommon.jl:52-124), 1 for Poisson (test/poisson/TestPoissonCommon.jl:18-41), and the
+    `density` attribute (default 1) of the other materials."""
+    if isinstance(physics, Mechanics):
+        return float(physics.props()[0])
+    return float(getattr(physics, "density", 1.0))
+
+
 # ------------------------------------------------------------------ function spaces and DOFs
 _ELEM_KIND = {"QUAD4": _lib.QUAD4, "HEX8": _lib.HEX8, "TET10": _lib.TET10}
 
@@ -262,6 +291,7 @@ class SparseMatrixAssembler:
         self.stiffness_action_storage = torch.zeros(N, dtype=f64, device=dev)
         self.stiffness_action_unknowns = torch.zeros(nu, dtype=f64, device=dev)
         self.stiffness_storage = torch.zeros(0 if self.matrix_free else self.nnz, dtype=f64, device=dev)
+        self.mass_storage = torch.zeros(0 if self.matrix_free else self.nnz, dtype=f64, device=dev)
 
     def pattern(self):
         """(rowptr, colval) 0-based int64 tensors on the GPU, bit-exact with sparse! + SparseMatrixCSR."""
@@ -293,6 +323,12 @@ class SparseMatrixAssembler:
             raise FECXError(-2, "stiffness is not available on a matrix-free assembler")
         rp, cv = self.pattern()
         return CSRMatrix(len(self.dof.unknown_dofs), rp, cv, self.stiffness_storage)
+
+    def _mass(self):
+        if self.matrix_free:
+            raise FECXError(-2, "mass is not available on a matrix-free assembler")
+        rp, cv = self.pattern()
+        return CSRMatrix(len(self.dof.unknown_dofs), rp, cv, self.mass_storage)
 
     def hvp(self, v):
         self.ctx.call("fecx_extract_field_unknowns", _ptr(self.stiffness_action_storage),
@@ -382,12 +418,44 @@ def assemble_vector_and_stiffness_(asm, Uu, p):
 
 
 def assemble_matrix_action_(asm, func, Uu, Vu, p):
-    """assemble_matrix_action! (src/assemblers/MatrixAction.jl:154-238)."""
+    """assemble_matrix_action! (src/assemblers/MatrixAction.jl:154-238); func = stiffness or mass."""
     _check_vec(asm, Uu, "Uu")
     _check_vec(asm, Vu, "Vu")
     _update_for_assembly(p, asm, Uu, Vu)
+    if func is mass:
+        asm.ctx.call("fecx_assemble_mass_action", ctypes.c_double(density(p.physics)), _ptr(p.hvp_scratch_field),
+                     _ptr(asm.stiffness_action_storage))
+        return
     asm.ctx.call("fecx_assemble_matrix_action", _ptr(p.field), _ptr(p.hvp_scratch_field),
                  _ptr(asm.stiffness_action_storage))
+
+
+def assemble_mass_(asm, func, Uu, p):
+    """assemble_mass! (src/assemblers/Matrix.jl:1-10): the consistent mass into mass_storage (CSR values)."""
+    if asm.matrix_free:
+        raise FECXError(-2, "assemble_mass! called on a matrix-free SparseMatrixAssembler. Re-create the assembler "
+                            "with matrix_free=False to enable matrix assembly.")
+    _check_vec(asm, Uu, "Uu")
+    _update_for_assembly(p, asm, Uu)
+    asm.ctx.call("fecx_assemble_mass", ctypes.c_double(density(p.physics)), _ptr(asm.mass_storage))
+
+
+def assemble_lumped_mass_(asm, func, Uu, p):
+    """assemble_lumped_mass! (src/assemblers/LumpedMass.jl:29-58): the row-sum lumped mass into residual_storage."""
+    _check_vec(asm, Uu, "Uu")
+    _update_for_assembly(p, asm, Uu)
+    asm.ctx.call("fecx_assemble_lumped_mass", ctypes.c_double(density(p.physics)), _ptr(asm.residual_storage))
+
+
+def assemble_diagonal_(asm, func, Uu, p):
+    """assemble_diagonal! (src/assemblers/Diagonal.jl:17-68): the diagonal of the stiffness (func = stiffness) or
+    of the consistent mass (func = mass) into residual_storage."""
+    _check_vec(asm, Uu, "Uu")
+    _update_for_assembly(p, asm, Uu)
+    if func is mass:
+        asm.ctx.call("fecx_assemble_mass_diagonal", ctypes.c_double(density(p.physics)), _ptr(asm.residual_storage))
+    else:
+        asm.ctx.call("fecx_assemble_diagonal", _ptr(p.field), _ptr(asm.residual_storage))
 
 
 def assemble_matrix_free_action_(asm, func_action, Uu, Vu, p):

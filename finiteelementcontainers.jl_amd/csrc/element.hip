@@ -24,15 +24,15 @@ struct ElemLDS {
   using T = ET<KIND>;
   static constexpr bool DO_R = (MODE & MODE_R) != 0, DO_K = (MODE & MODE_K) != 0;
   static constexpr bool DO_MF = (MODE & MODE_ACTION_MF) != 0, DO_ASM = (MODE & MODE_ACTION_ASM) != 0;
-  static constexpr bool NEED_C = DO_K || DO_MF || DO_ASM;
+  static constexpr bool DO_D = (MODE & MODE_DIAG) != 0;
+  static constexpr bool NEED_C = DO_K || DO_MF || DO_ASM || DO_D;
   static constexpr int D = NF * T::ND;
   float x[T::NNPE][T::ND];
   float x0[T::ND];
   float u[T::NNPE][NF];
   float v[(DO_MF || DO_ASM) ? T::NNPE : 1][NF];
-  float kv[DO_ASM ? T::NNPE : 1][NF];
   struct QP {
-    float v[T::NNPE][(DO_K || DO_ASM) ? Mat<PHYS>::NV : T::ND];  // v[a][0..ND) = shape gradient g_a
+    float v[T::NNPE][(DO_K || DO_ASM || DO_D) ? Mat<PHYS>::NV : T::ND];  // v[a][0..ND) = shape gradient g_a
     float w;
     float P[(DO_R || DO_MF) ? D : 1];
     float src[(DO_R && PHYS == FECX_POISSON) ? 1 : 1];
@@ -41,7 +41,7 @@ struct ElemLDS {
   // the element matrix blocks are staged over the quadrature data once that has been consumed
   union {
     QP qp[T::NQ];
-    float K[DO_K ? T::NPAIRS * NF * NF : 1];
+    float K[(DO_K || DO_ASM) ? T::NPAIRS * NF * NF : 1];
   };
 };
 
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   using LDS = ElemLDS<KIND, NF, PHYS, MODE>;
   constexpr int ND = T::ND, NNPE = T::NNPE, NQ = T::NQ, G = T::G, EPW = ELEM_THREADS / G, NDOF = NNPE * NF;
   constexpr int D = NF * ND;
-  constexpr bool DO_R = LDS::DO_R, DO_K = LDS::DO_K, DO_MF = LDS::DO_MF, DO_ASM = LDS::DO_ASM;
+  constexpr bool DO_R = LDS::DO_R, DO_K = LDS::DO_K, DO_MF = LDS::DO_MF, DO_ASM = LDS::DO_ASM, DO_D = LDS::DO_D;
   constexpr bool DO_G = (MODE & MODE_QPGRAD) != 0;
   constexpr bool NEED_C = LDS::NEED_C;
   __shared__ LDS lds[EPW];
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         if (lane < NNPE) L.v[lane][i] = (float)(vv[i] - v0);
       }
       if constexpr (DO_ASM) {
-        if (lane < NNPE) { L.v[lane][i] = (float)vv[i]; L.kv[lane][i] = 0.0f; }
+        if (lane < NNPE) L.v[lane][i] = (float)vv[i];
       }
     }
   }
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         Q.src[0] = f;
       }
     }
-    if constexpr (DO_K || DO_ASM) {
+    if constexpr (DO_K || DO_ASM || DO_D) {
 #pragma unroll
       for (int c = 0; c < M::NS; ++c) Q.S[c] = C[M::S0 + c];
 #pragma unroll
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
 
   // ---------------- node rows (lane = a)
   constexpr int MAXP = T::MAXP;
-  float kreg[DO_K ? MAXP : 1][NF * NF];
+  float kreg[(DO_K || DO_ASM) ? MAXP : 1][NF * NF];
   if (valid && lane < NNPE) {
     const int a = lane;
     if constexpr (DO_R || DO_MF) {
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
             acc[0][k * NF + i] = m;
           }
       }
-      if constexpr (DO_K) {
+      if constexpr (DO_K || DO_ASM) {
         // every lane of the work-group has finished reading the quadrature data of its element only after the
         // barrier below, so the blocks are kept in registers until then
 #pragma unroll
@@ -312,37 +312,55 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
 #pragma unroll
           for (int c = 0; c < NF * NF; ++c) kreg[p][c] = acc[p][c];
       }
-      if constexpr (DO_ASM) {
+    }
+  }
+  if constexpr (DO_D) {
+    // diagonal of the element Jacobian (the AssembledDiagonal return type): lane a forms its own block K_aa only
+    if (valid && lane < NNPE) {
+      const int a = lane;
+      float acc[NF * NF] = {};
+#pragma unroll 2
+      for (int q = 0; q < NQ; ++q) {
+        const auto& Q = L.qp[q];
+        float C[M::NC];
 #pragma unroll
-        for (int p = 0; p < MAXP; ++p) {
-          if (p < cnt) {
-            const int b = a + p < NNPE ? a + p : a + p - NNPE;
+        for (int c = 0; c < M::NS; ++c) C[M::S0 + c] = Q.S[c];
+        float va[M::NV], sa[M::NA];
 #pragma unroll
-            for (int i = 0; i < NF; ++i) {
-              float s = 0;
-#pragma unroll
-              for (int k = 0; k < NF; ++k) s += acc[p][i * NF + k] * L.v[b][k];
-              atomicAdd(&L.kv[a][i], s);
-            }
-            if (b != a) {
-#pragma unroll
-              for (int k = 0; k < NF; ++k) {
-                float s = 0;
-#pragma unroll
-                for (int i = 0; i < NF; ++i) s += acc[p][i * NF + k] * L.v[a][i];
-                atomicAdd(&L.kv[b][k], s);
-              }
-            }
-          }
-        }
+        for (int c = 0; c < M::NV; ++c) va[c] = Q.v[a][c];
+        M::template side<ND>(args.mp, C, Q.w, va, sa);
+        M::template pair<ND>(sa, va, acc);
       }
+#pragma unroll
+      for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + a * NF + i] = acc[i * NF + i];
     }
   }
   if constexpr (DO_ASM) {
+    // element action K_el v_el without atomics: stage the element's symmetric blocks in LDS (over the consumed
+    // quadrature data), then lane a sums its row K_ab v_b over b in ascending order
     __syncthreads();
     if (valid && lane < NNPE) {
+      const int cnt = lane_cnt(NNPE, lane);
+      float* dst = L.K + pair_lane_off(NNPE, lane) * (NF * NF);
 #pragma unroll
-      for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + lane * NF + i] = L.kv[lane][i];
+      for (int p = 0; p < MAXP; ++p)
+        if (p < cnt)
+#pragma unroll
+          for (int c = 0; c < NF * NF; ++c) dst[p * NF * NF + c] = kreg[p][c];
+    }
+    __syncthreads();
+    if (valid && lane < NNPE) {
+      float kv[NF] = {};
+      for (int b = 0; b < NNPE; ++b) {
+        const PairRef pr = pair_ref(NNPE, lane, b);
+        const float* blk = L.K + pr.slot * (NF * NF);
+#pragma unroll
+        for (int i = 0; i < NF; ++i)
+#pragma unroll
+          for (int k = 0; k < NF; ++k) kv[i] = fmaf(pr.ab ? blk[i * NF + k] : blk[k * NF + i], L.v[b][k], kv[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + lane * NF + i] = kv[i];
     }
   }
   if constexpr (DO_K) {
@@ -386,6 +404,7 @@ static void launch_modes(int mode, const ElementArgs& a, hipStream_t s) {
     case MODE_R | MODE_K: element_kernel<KIND, NF, PHYS, MODE_R | MODE_K><<<grid, B, 0, s>>>(a); break;
     case MODE_ACTION_MF: element_kernel<KIND, NF, PHYS, MODE_ACTION_MF><<<grid, B, 0, s>>>(a); break;
     case MODE_ACTION_ASM: element_kernel<KIND, NF, PHYS, MODE_ACTION_ASM><<<grid, B, 0, s>>>(a); break;
+    case MODE_DIAG: element_kernel<KIND, NF, PHYS, MODE_DIAG><<<grid, B, 0, s>>>(a); break;
     case MODE_QPGRAD:  // gradients do not depend on the physics: one instantiation per (KIND, NF)
       element_kernel<KIND, NF, NF == 1 ? FECX_POISSON : FECX_NEOHOOKEAN, MODE_QPGRAD><<<grid, B, 0, s>>>(a);
       break;

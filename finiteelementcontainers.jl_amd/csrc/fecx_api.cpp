@@ -141,7 +141,7 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
     HIPCHK(hipGetLastError());
   }
   fecx::ReduceArgs ra = reduce_args(ctx);
-  if (mode & (fecx::MODE_R | fecx::MODE_ACTION_MF | fecx::MODE_ACTION_ASM)) ra.R = R;
+  if (mode & (fecx::MODE_R | fecx::MODE_ACTION_MF | fecx::MODE_ACTION_ASM | fecx::MODE_DIAG)) ra.R = R;
   if (mode & fecx::MODE_K) ra.vals = vals;
   const int k = pipeline_chunks(ctx, mode);
   hipStream_t es = ctx->stream;
@@ -180,6 +180,34 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   int hflag = 0;
   HIPCHK(hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost));
   require(hflag == 0, FECX_ERR_NUMERIC, "element kernel met a non-positive Jacobian determinant or a NaN");
+}
+
+// Mass-type assembly: the mass kernel for `what`, then the reduction into the CSR values (MASS_MATRIX) or the full
+// field R (the others). Needs the mesh, the DOFs and the pattern, not a material.
+void assemble_mass(fecx_ctx* ctx, int what, double density, const double* V, double* R, double* vals) {
+  check_device(ctx);
+  require(ctx->has_mesh && ctx->has_dofs && ctx->has_pattern, FECX_ERR_STATE,
+          "mesh, dofs and pattern must be set before a mass assembly");
+  require(std::isfinite(density), FECX_ERR_ARG, "density must be finite");
+  Prof prof(ctx);
+  int* flag = get<int>(ctx, "flag");
+  zero_int_kernel<<<1, 1, 0, ctx->stream>>>(flag);
+  fecx::ElementArgs ea = element_args(ctx, nullptr, V);
+  int rc = 0;
+  prof.run("mass_element", ctx->stream, [&] { rc = fecx::launch_mass(ctx->kind, ctx->nf, what, ea, (float)density, ctx->stream); });
+  require(rc == 0, FECX_ERR_UNSUPPORTED, "no mass kernel for this element kind / nf");
+  HIPCHK(hipGetLastError());
+  fecx::ReduceArgs ra = reduce_args(ctx);
+  ra.R = R;
+  ra.vals = vals;
+  prof.run("reduce", ctx->stream, [&] { rc = fecx::launch_reduce(ctx->nf, ra, ctx->stream); });
+  require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  prof.finish();
+  int hflag = 0;
+  HIPCHK(hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost));
+  require(hflag == 0, FECX_ERR_NUMERIC, "mass kernel met a non-positive Jacobian determinant");
 }
 
 }  // namespace
@@ -471,14 +499,14 @@ int fecx_assemble_vector(fecx_ctx* ctx, const double* U, double* R) {
 
 int fecx_assemble_matrix(fecx_ctx* ctx, const double* U, double* vals) {
   return guard(ctx, [&] {
-    require(vals != nullptr, FECX_ERR_ARG, "vals is NULL");
+    require(vals != nullptr || ctx->nnz == 0, FECX_ERR_ARG, "vals is NULL");
     assemble(ctx, fecx::MODE_K, U, nullptr, nullptr, vals);
   });
 }
 
 int fecx_assemble_vector_matrix(fecx_ctx* ctx, const double* U, double* R, double* vals) {
   return guard(ctx, [&] {
-    require(R != nullptr && vals != nullptr, FECX_ERR_ARG, "R or vals is NULL");
+    require(R != nullptr && (vals != nullptr || ctx->nnz == 0), FECX_ERR_ARG, "R or vals is NULL");
     assemble(ctx, fecx::MODE_R | fecx::MODE_K, U, nullptr, R, vals);
   });
 }
@@ -494,6 +522,41 @@ int fecx_assemble_matrix_free_action(fecx_ctx* ctx, const double* U, const doubl
   return guard(ctx, [&] {
     require(V != nullptr && KV != nullptr, FECX_ERR_ARG, "V or KV is NULL");
     assemble(ctx, fecx::MODE_ACTION_MF, U, V, KV, nullptr);
+  });
+}
+
+int fecx_assemble_diagonal(fecx_ctx* ctx, const double* U, double* D) {
+  return guard(ctx, [&] {
+    require(D != nullptr, FECX_ERR_ARG, "D is NULL");
+    assemble(ctx, fecx::MODE_DIAG, U, nullptr, D, nullptr);
+  });
+}
+
+int fecx_assemble_mass(fecx_ctx* ctx, double density, double* vals) {
+  return guard(ctx, [&] {
+    require(vals != nullptr || ctx->nnz == 0, FECX_ERR_ARG, "vals is NULL");
+    assemble_mass(ctx, fecx::MASS_MATRIX, density, nullptr, nullptr, vals);
+  });
+}
+
+int fecx_assemble_lumped_mass(fecx_ctx* ctx, double density, double* M) {
+  return guard(ctx, [&] {
+    require(M != nullptr, FECX_ERR_ARG, "M is NULL");
+    assemble_mass(ctx, fecx::MASS_LUMPED, density, nullptr, M, nullptr);
+  });
+}
+
+int fecx_assemble_mass_diagonal(fecx_ctx* ctx, double density, double* D) {
+  return guard(ctx, [&] {
+    require(D != nullptr, FECX_ERR_ARG, "D is NULL");
+    assemble_mass(ctx, fecx::MASS_DIAG, density, nullptr, D, nullptr);
+  });
+}
+
+int fecx_assemble_mass_action(fecx_ctx* ctx, double density, const double* V, double* MV) {
+  return guard(ctx, [&] {
+    require(V != nullptr && MV != nullptr, FECX_ERR_ARG, "V or MV is NULL");
+    assemble_mass(ctx, fecx::MASS_ACTION, density, V, MV, nullptr);
   });
 }
 

@@ -91,7 +91,10 @@ struct MatParams {
   float p[8];
 };
 
-enum Mode { MODE_R = 1, MODE_K = 2, MODE_ACTION_MF = 4, MODE_ACTION_ASM = 8, MODE_QPGRAD = 16 };
+enum Mode { MODE_R = 1, MODE_K = 2, MODE_ACTION_MF = 4, MODE_ACTION_ASM = 8, MODE_QPGRAD = 16, MODE_DIAG = 32 };
+
+// mass-type assemblies (mass.hip): consistent mass blocks into Kel; lumped mass, mass diagonal or mass action into Rel
+enum MassWhat { MASS_MATRIX = 1, MASS_LUMPED = 2, MASS_DIAG = 3, MASS_ACTION = 4 };
 
 struct ElementArgs {
   int64_t ebeg, ne;         // element storage positions [ebeg, ne)
@@ -113,6 +116,8 @@ struct ElementArgs {
 
 // returns 0, or -1 if the (kind, nf, phys) combination has no kernel
 int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t s);
+// returns 0, or -1 if the (kind, nf) combination has no mass kernel; rho multiplies every weight
+int launch_mass(int kind, int nf, int what, const ElementArgs& a, float rho, hipStream_t s);
 
 struct ReduceArgs {
   int64_t nn;

@@ -16,6 +16,9 @@ namespace fecx {
 
 constexpr int RED_THREADS = 256;
 constexpr int CH = 8;            // incidences staged per pass
+#ifndef RED_LB
+#define RED_LB 4                 // vector rows: incidences whose block loads are in flight together
+#endif
 
 // Work-groups are dealt to the 8 XCDs round-robin; remap so that each XCD walks a contiguous range of nodes,
 // which keeps the element blocks shared by neighbouring nodes in that XCD's L2.
@@ -99,14 +102,56 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
     }
     __syncthreads();
     if (r >= 0) {
-      for (int t = 0; t < cnt; ++t) {
-        const uint32_t v = (uint32_t)s_v[grp][t];
-        const uint32_t ebase = (v / NNPE) * (uint32_t)EW;  // element storage is limited to 2^31 words
-        const int la = (int)(v % NNPE);
+      if constexpr (NF == 1) {
+        // scalar rows: 8 words per incidence, one load per lane and incidence
+        for (int t = 0; t < cnt; ++t) {
+          const uint32_t v = (uint32_t)s_v[grp][t];
+          const uint32_t ebase = (v / NNPE) * (uint32_t)EW;  // element storage is limited to 2^31 words
+          const int la = (int)(v % NNPE);
 #pragma unroll
-        for (int j0 = 0; j0 < BW; j0 += LPR) {
-          const int j = j0 + lane;
-          if (j < BW) s_blk[grp][t][j] = a.Kel[ebase + (uint32_t)s_off[la][j]];
+          for (int j0 = 0; j0 < BW; j0 += LPR) {
+            const int j = j0 + lane;
+            if (j < BW) s_blk[grp][t][j] = a.Kel[ebase + (uint32_t)s_off[la][j]];
+          }
+        }
+      } else {
+        // vector rows: incidences in batches of LB -- all word indices of a batch first, then its loads back to back
+        // (their latencies overlap), then the LDS writes. Slots past the row's incidences re-read the first
+        // incidence's base word and are never written. LB = 4 keeps the kernel near 80 VGPRs (6 waves per SIMD);
+        // 8 in flight needs ~106 and measured slower.
+        constexpr int JN = (BW + LPR - 1) / LPR;
+        constexpr int LB = RED_LB;
+#pragma unroll
+        for (int t0 = 0; t0 < CH; t0 += LB) {
+          uint32_t idx[LB][JN];
+#pragma unroll
+          for (int tt = 0; tt < LB; ++tt) {
+            const int t = t0 + tt;
+            const uint32_t v = (uint32_t)s_v[grp][t < cnt ? t : 0];
+            const uint32_t ebase = (v / NNPE) * (uint32_t)EW;
+            const int la = (int)(v % NNPE);
+#pragma unroll
+            for (int jj = 0; jj < JN; ++jj) {
+              const int j = jj * LPR + lane;
+              idx[tt][jj] = (t < cnt && j < BW) ? ebase + (uint32_t)s_off[la][j] : ebase;
+            }
+          }
+          float tmp[LB][JN];
+#pragma unroll
+          for (int tt = 0; tt < LB; ++tt)
+#pragma unroll
+            for (int jj = 0; jj < JN; ++jj) tmp[tt][jj] = a.Kel[idx[tt][jj]];
+#pragma unroll
+          for (int tt = 0; tt < LB; ++tt) {
+            const int t = t0 + tt;
+            if (t < cnt) {
+#pragma unroll
+              for (int jj = 0; jj < JN; ++jj) {
+                const int j = jj * LPR + lane;
+                if (j < BW) s_blk[grp][t][j] = tmp[tt][jj];
+              }
+            }
+          }
         }
       }
     }

@@ -155,6 +155,27 @@ int fecx_assemble_matrix_action(fecx_ctx* ctx, const double* U, const double* V,
  * Replaces assemble_matrix_free_action! / _full! (src/assemblers/MatrixAction.jl:9-77,99-149). */
 int fecx_assemble_matrix_free_action(fecx_ctx* ctx, const double* U, const double* V, double* KV);
 
+/* Element-Jacobian diagonals at U into the full field D (zeroed first); diagonal(asm) is D at the unknowns.
+ * Replaces assemble_diagonal!(asm, stiffness, Uu, p) (src/assemblers/Diagonal.jl:17-68, the AssembledDiagonal
+ * accumulation Assemblers.jl:42-45). */
+int fecx_assemble_diagonal(fecx_ctx* ctx, const double* U, double* D);
+
+/* Mass assemblies with the test physics' mass callbacks M_el = density JxW kron(N N^T, I_NF) and
+ * m_el = density JxW N_a (test/mechanics/TestMechanicsCommon.jl:52-124; density = props[1] for Mechanics, 1 for
+ * Poisson, test/poisson/TestPoissonCommon.jl:18-41). They need no material and no field.
+ * fecx_assemble_mass: CSR values (nnz) of the consistent mass on the Jacobian's pattern; replaces assemble_mass!
+ *   followed by mass(asm) (src/assemblers/Matrix.jl:1-10, Assemblers.jl:253-336).
+ * fecx_assemble_lumped_mass: row-sum lumped mass into the full field M; replaces assemble_lumped_mass! and
+ *   lumped_mass(asm) (src/assemblers/LumpedMass.jl:29-80).
+ * fecx_assemble_mass_diagonal: diagonal of the consistent mass into the full field D; replaces
+ *   assemble_diagonal!(asm, mass, Uu, p) (src/assemblers/Diagonal.jl:17-68).
+ * fecx_assemble_mass_action: M V into the full field MV; replaces assemble_matrix_action!(asm, mass, Uu, Vu, p)
+ *   (src/assemblers/MatrixAction.jl:154-238). */
+int fecx_assemble_mass(fecx_ctx* ctx, double density, double* vals);
+int fecx_assemble_lumped_mass(fecx_ctx* ctx, double density, double* M);
+int fecx_assemble_mass_diagonal(fecx_ctx* ctx, double density, double* D);
+int fecx_assemble_mass_action(fecx_ctx* ctx, double density, const double* V, double* MV);
+
 /* U[unknown_dofs] = Uu: update_field_unknowns! (src/DofManagers.jl:328-411). */
 int fecx_update_field_unknowns(fecx_ctx* ctx, const double* Uu, double* U);
 

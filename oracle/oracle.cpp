@@ -18,6 +18,10 @@
 //   * neo-Hookean           : P(I)=0, P = dpsi/dF and A = dP/dF by central differences
 //   * J2 radial return      : yield consistency, elastic predictor, FD of the update,
 //                             uniaxial closed form
+//   * mass, lumped mass     : closed-form quad4/hex8 element mass matrices, tet10 vertex/edge
+//     and diagonals           lumped masses -V/20 and V/5, lumped = row sums of the consistent
+//                             mass (test/TestAssemblers.jl:432-521), total mass = rho V,
+//                             diagonal = diag of the sparse! CSR, mass action = CSR x v
 //   * tet10 node order and the quadrature rule are conventions of this build
 //     ("parity unpinned" against the reference's ReferenceFiniteElements.jl).
 //
@@ -364,6 +368,43 @@ void element(const Problem& pr, i64 e, const double* U, const double* V, const d
   }
 }
 
+// one element's consistent mass M_el[(a,i),(b,k)] = rho sum_q JxW N_a N_b delta_ik (NDOF x NDOF row-major):
+// the test physics' mass callbacks (test/mechanics/TestMechanicsCommon.jl:52-85 with rho = props[1],
+// test/poisson/TestPoissonCommon.jl:18-41 with rho = 1)
+void mass_element(const Problem& pr, i64 e, double rho, double* Mel) {
+  const int nd = pr.nd, nf = pr.nf, nn = pr.nnpe, NDOF = nn * nf;
+  double xel[30];
+  for (int a = 0; a < nn; ++a) {
+    i64 node = pr.conn[e * nn + a] - 1;
+    for (int d = 0; d < nd; ++d) xel[a * nd + d] = pr.X[node * nd + d];
+  }
+  for (int i = 0; i < NDOF * NDOF; ++i) Mel[i] = 0;
+  for (int q = 0; q < pr.re.nq; ++q) {
+    Mapped m;
+    map_qp(pr.kind, pr.re, q, xel, m);
+    for (int a = 0; a < nn; ++a)
+      for (int b = 0; b < nn; ++b)
+        for (int i = 0; i < nf; ++i) Mel[(a * nf + i) * NDOF + b * nf + i] += rho * m.JxW * m.N[a] * m.N[b];
+  }
+}
+
+// one element's row-sum lumped mass m_el[(a,i)] = rho sum_q JxW N_a (TestMechanicsCommon.jl:98-124)
+void lumped_element(const Problem& pr, i64 e, double rho, double* mel) {
+  const int nd = pr.nd, nf = pr.nf, nn = pr.nnpe;
+  double xel[30];
+  for (int a = 0; a < nn; ++a) {
+    i64 node = pr.conn[e * nn + a] - 1;
+    for (int d = 0; d < nd; ++d) xel[a * nd + d] = pr.X[node * nd + d];
+  }
+  for (int i = 0; i < nn * nf; ++i) mel[i] = 0;
+  for (int q = 0; q < pr.re.nq; ++q) {
+    Mapped m;
+    map_qp(pr.kind, pr.re, q, xel, m);
+    for (int a = 0; a < nn; ++a)
+      for (int i = 0; i < nf; ++i) mel[a * nf + i] += rho * m.JxW * m.N[a];
+  }
+}
+
 inline void atomic_add(double* p, double v) {
 #pragma omp atomic
   *p += v;
@@ -650,6 +691,93 @@ void orc_assemble_action(int elem_kind, int nd, i64 nn, const double* X, i64 ne,
       }
       for (int d = 0; d < nf; ++d)
         for (int a = 0; a < pr.nnpe; ++a) atomic_add(&KV[nf * (conn[e * pr.nnpe + a] - 1) + d], out[a * nf + d]);
+    }
+  }
+}
+
+// Consistent mass element storage (assemble_mass!, src/assemblers/Matrix.jl:1-10 -> assemble_matrix!), laid out
+// like orc_assemble_stiffness_storage (M_el is symmetric, so its transpose is itself).
+void orc_assemble_mass_storage(int elem_kind, int nd, i64 nn, const double* X, i64 ne, const i64* conn, int nf,
+                               double rho, double* storage) {
+  Problem pr = make_problem(elem_kind, nd, nn, X, ne, conn, nf, POISSON, nullptr, 0);
+  const int NDOF = pr.nnpe * nf;
+#pragma omp parallel
+  {
+    std::vector<double> Mel(NDOF * NDOF);
+#pragma omp for schedule(static)
+    for (i64 e = 0; e < ne; ++e) {
+      mass_element(pr, e, rho, Mel.data());
+      double* s = storage + e * NDOF * NDOF;
+      for (int r = 0; r < NDOF; ++r)
+        for (int c = 0; c < NDOF; ++c) s[r * NDOF + c] = Mel[c * NDOF + r];
+    }
+  }
+}
+
+// Row-sum lumped mass into a full NF x NN field (assemble_lumped_mass!, src/assemblers/LumpedMass.jl:29-58).
+void orc_assemble_lumped_mass(int elem_kind, int nd, i64 nn, const double* X, i64 ne, const i64* conn, int nf,
+                              double rho, double* M) {
+  Problem pr = make_problem(elem_kind, nd, nn, X, ne, conn, nf, POISSON, nullptr, 0);
+  const int NDOF = pr.nnpe * nf;
+  for (i64 i = 0; i < nn * nf; ++i) M[i] = 0;
+#pragma omp parallel
+  {
+    std::vector<double> mel(NDOF);
+#pragma omp for schedule(static)
+    for (i64 e = 0; e < ne; ++e) {
+      lumped_element(pr, e, rho, mel.data());
+      for (int d = 0; d < nf; ++d)
+        for (int a = 0; a < pr.nnpe; ++a) atomic_add(&M[nf * (conn[e * pr.nnpe + a] - 1) + d], mel[a * nf + d]);
+    }
+  }
+}
+
+// Element diagonals scattered into a full NF x NN field (assemble_diagonal!, src/assemblers/Diagonal.jl:17-68, the
+// AssembledDiagonal accumulation Assemblers.jl:42-45): which = 0 the stiffness of (phys, props) at U, which = 1 the
+// consistent mass with density rho.
+void orc_assemble_diagonal(int elem_kind, int nd, i64 nn, const double* X, i64 ne, const i64* conn, int nf, int phys,
+                           const double* props, int nprops, const double* U, const double* st_old, int which,
+                           double rho, double* Dg) {
+  Problem pr = make_problem(elem_kind, nd, nn, X, ne, conn, nf, phys, props, nprops);
+  const int NDOF = pr.nnpe * nf;
+  for (i64 i = 0; i < nn * nf; ++i) Dg[i] = 0;
+#pragma omp parallel
+  {
+    std::vector<double> Kel(NDOF * NDOF);
+#pragma omp for schedule(static)
+    for (i64 e = 0; e < ne; ++e) {
+      if (which == 1) mass_element(pr, e, rho, Kel.data());
+      else element(pr, e, U, nullptr, st_old, nullptr, nullptr, Kel.data(), nullptr);
+      for (int d = 0; d < nf; ++d)
+        for (int a = 0; a < pr.nnpe; ++a) {
+          const int r = a * nf + d;
+          atomic_add(&Dg[nf * (conn[e * pr.nnpe + a] - 1) + d], Kel[r * NDOF + r]);
+        }
+    }
+  }
+}
+
+// Mass action M v into a full NF x NN field (assemble_matrix_action! with the mass callback,
+// src/assemblers/MatrixAction.jl:154-238).
+void orc_assemble_mass_action(int elem_kind, int nd, i64 nn, const double* X, i64 ne, const i64* conn, int nf,
+                              double rho, const double* V, double* MV) {
+  Problem pr = make_problem(elem_kind, nd, nn, X, ne, conn, nf, POISSON, nullptr, 0);
+  const int NDOF = pr.nnpe * nf;
+  for (i64 i = 0; i < nn * nf; ++i) MV[i] = 0;
+#pragma omp parallel
+  {
+    std::vector<double> Mel(NDOF * NDOF), vel(NDOF);
+#pragma omp for schedule(static)
+    for (i64 e = 0; e < ne; ++e) {
+      mass_element(pr, e, rho, Mel.data());
+      for (int a = 0; a < pr.nnpe; ++a)
+        for (int i = 0; i < nf; ++i) vel[a * nf + i] = V[nf * (conn[e * pr.nnpe + a] - 1) + i];
+      for (int r = 0; r < NDOF; ++r) {
+        double s = 0;
+        for (int c = 0; c < NDOF; ++c) s += Mel[r * NDOF + c] * vel[c];
+        const int a = r / nf, d = r % nf;
+        atomic_add(&MV[nf * (conn[e * pr.nnpe + a] - 1) + d], s);
+      }
     }
   }
 }

@@ -146,6 +146,53 @@ def assemble_action(elem_kind, X, conn, nf, phys, props, U, V, state_old=None, m
     return KV
 
 
+def assemble_mass_storage(elem_kind, X, conn, nf, rho):
+    X, conn, _, nn, nd, ne = _common(elem_kind, X, conn, nf, 0, [])
+    ndof = conn.shape[1] * nf
+    st = np.zeros(ne * ndof * ndof)
+    lib().orc_assemble_mass_storage(ctypes.c_int(elem_kind), ctypes.c_int(nd), _i64(nn), _p(X), _i64(ne), _p(conn),
+                                    ctypes.c_int(nf), ctypes.c_double(rho), _p(st))
+    return st
+
+
+def assemble_lumped_mass(elem_kind, X, conn, nf, rho):
+    X, conn, _, nn, nd, ne = _common(elem_kind, X, conn, nf, 0, [])
+    M = np.zeros(nn * nf)
+    lib().orc_assemble_lumped_mass(ctypes.c_int(elem_kind), ctypes.c_int(nd), _i64(nn), _p(X), _i64(ne), _p(conn),
+                                   ctypes.c_int(nf), ctypes.c_double(rho), _p(M))
+    return M
+
+
+def assemble_diagonal(elem_kind, X, conn, nf, phys, props, U, state_old=None, which="stiffness", rho=1.0):
+    """Full-field element diagonals of the stiffness (at U) or of the consistent mass (density rho)."""
+    X, conn, props, nn, nd, ne = _common(elem_kind, X, conn, nf, phys, props)
+    U = np.ascontiguousarray(U if U is not None else np.zeros(nn * nf), dtype=np.float64)
+    Dg = np.zeros(nn * nf)
+    lib().orc_assemble_diagonal(ctypes.c_int(elem_kind), ctypes.c_int(nd), _i64(nn), _p(X), _i64(ne), _p(conn),
+                                ctypes.c_int(nf), ctypes.c_int(phys), _p(props), ctypes.c_int(len(props)), _p(U),
+                                _p(state_old), ctypes.c_int(1 if which == "mass" else 0), ctypes.c_double(rho),
+                                _p(Dg))
+    return Dg
+
+
+def assemble_mass_action(elem_kind, X, conn, nf, rho, V):
+    X, conn, _, nn, nd, ne = _common(elem_kind, X, conn, nf, 0, [])
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    MV = np.zeros(nn * nf)
+    lib().orc_assemble_mass_action(ctypes.c_int(elem_kind), ctypes.c_int(nd), _i64(nn), _p(X), _i64(ne), _p(conn),
+                                   ctypes.c_int(nf), ctypes.c_double(rho), _p(V), _p(MV))
+    return MV
+
+
+def mass_csr(elem_kind, X, conn, nf, rho, dof_to_unknown, n_unknowns):
+    """The reference's mass(asm) for the CSR type: consistent mass storage -> sparse! -> CSR."""
+    Is, Js, slots, perm = pattern(nf, conn, dof_to_unknown)
+    st = assemble_mass_storage(elem_kind, X, conn, nf, rho)
+    V = gather_slots(slots, st)
+    rowptr, colval, vals, _ = sparse_csr(n_unknowns, Is, Js, V)
+    return rowptr, colval, vals
+
+
 def gather_slots(slots, storage):
     slots = np.ascontiguousarray(slots, dtype=np.int64)
     out = np.zeros(len(slots))

@@ -5,7 +5,7 @@ checked at bounded sizes elsewhere):
   - Poisson: K 1 = 0 on an all-free mesh (constants are in the null space); the residual is affine in u:
     R(u1 + u2) = R(u1) + R(u2) - R(0);
   - the matrix-free action and the assembled action both equal K v (SpMV of the assembled Jacobian);
-  - assembly is deterministic: two assemblies give the same bits (no atomics anywhere in the reduction).
+  - assembly and the assembled action are deterministic: repeated calls give the same bits (no atomics).
 Tolerances (relative to the norms involved): symmetry 1e-12 (FP64 SpMV of an exactly symmetric matrix, summation
 order only), null space 1e-6 (FP32 element sums), linearity 1e-6, actions 2e-6."""
 import ctypes
@@ -55,6 +55,8 @@ def test_full_size_properties(fec, gpu, name, n):
     mf = fec.hvp(asm, x).clone()
     fec.assemble_matrix_action_(asm, fec.stiffness, Uu, x, p)
     am = fec.hvp(asm, x).clone()
+    fec.assemble_matrix_action_(asm, fec.stiffness, Uu, x, p)
+    assert torch.equal(am, fec.hvp(asm, x)), "assembled action not deterministic"
     nk = float(torch.linalg.norm(Kx))
     e_mf, e_am = float(torch.linalg.norm(mf - Kx)) / nk, float(torch.linalg.norm(am - Kx)) / nk
     print(f"[fullsize] action vs K v: matrix-free {e_mf:.2e}, assembled {e_am:.2e}", flush=True)

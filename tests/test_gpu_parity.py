@@ -164,3 +164,44 @@ def test_pipelined_chunks_bit_identical(fec, gpu, name):
         out.append((fec.residual(asm).cpu().numpy().copy(), asm.stiffness_storage.cpu().numpy().copy()))
     for R, K in out[1:]:
         assert np.array_equal(R, out[0][0]) and np.array_equal(K, out[0][1])
+
+
+@pytest.mark.parametrize("case", ["one_hex8_nh", "one_tet10_poisson", "all_dirichlet", "one_free_node"])
+def test_edge_meshes(fec, orc, gpu, case):
+    """Smallest meshes and degenerate DOF sets: one element, every dof constrained (empty system), one unknown."""
+    print(f"[gpu edge] {case}", flush=True)
+    W = fec.workloads
+    if case == "one_hex8_nh":
+        mesh = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (2, 2, 2))
+        wl = W.Workload(case, mesh, fec.NeoHookean(1.0, 0.5), 3, 0.05 * np.sin(np.arange(24.0)))
+    elif case == "one_tet10_poisson":
+        v = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.2, 1.0, 0.0], [0.1, 0.2, 0.9]])
+        edges = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+        X = np.vstack([v] + [0.5 * (v[a] + v[b]) for a, b in edges])
+        mesh = fec.TetMesh(X, np.arange(1, 11, dtype=np.int64)[None, :])
+        wl = W.Workload(case, mesh, fec.Poisson("sines", 1.0), 1, np.cos(np.arange(10.0)))
+    else:
+        mesh = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (3, 3, 3))
+        dd = np.arange(1, 28, dtype=np.int64) if case == "all_dirichlet" else np.delete(np.arange(1, 28), 13)
+        wl = W.Workload(case, mesh, fec.Poisson("constant", 1.0), 1, np.linspace(-1, 1, 27), dd)
+    asm, p = _setup(fec, wl, gpu)
+    unknown = asm.dof.unknown_dofs
+    N = wl.nf * mesh.num_nodes
+    _, d2u = orc.update_dofs(N, asm.dof.dirichlet_dofs)
+    U = wl.U.copy()
+    U[asm.dof.dirichlet_dofs - 1] = 0.0
+    Vv = np.zeros(N)
+    Vv[unknown - 1] = 1.0 + np.arange(len(unknown))
+    R_ref, (rp, cv, vals), KVa_ref, KVm_ref = _oracle_all(orc, wl, U, Vv, unknown, d2u)
+    Uu = torch.from_numpy(U[unknown - 1]).to(gpu)
+    Vu = torch.from_numpy(Vv[unknown - 1]).to(gpu)
+    fec.assemble_vector_and_stiffness_(asm, Uu, p)
+    R = fec.residual(asm).cpu().numpy()
+    K = fec.stiffness(asm)
+    fec.assemble_matrix_free_action_(asm, fec.stiffness_action, Uu, Vu, p)
+    KV = fec.hvp(asm, Vu).cpu().numpy()
+    assert len(R) == len(unknown) and K.values.numel() == len(vals)
+    assert np.array_equal(K.rowptr.cpu().numpy() + 1, rp) and np.array_equal(K.colval.cpu().numpy() + 1, cv)
+    if len(unknown):
+        assert rel(R, R_ref) < TOL["R"] and rel(K.values.cpu().numpy(), vals) < TOL["K"]
+        assert rel(KV, KVm_ref) < TOL["KV"]

@@ -378,3 +378,72 @@ def test_field_seed_rule(fec, amp):
         if np.abs(a[3]).max() * amp * 4 * np.pi > 0.9:
             continue  # the k = 4 term alone folds the field
         assert W.sine_field_min_det(amp, t, pts=41) <= 0.2, t
+
+
+# ------------------------------------------------------------------ mass, lumped mass, diagonals
+def _single_tet10(v):
+    """One straight-sided tet10 (Exodus TETRA10 node order) on vertices v (4 x 3)."""
+    edges = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+    X = np.vstack([v] + [0.5 * (v[a] + v[b]) for a, b in edges])
+    return X, np.arange(1, 11, dtype=np.int64)[None, :]
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_mass_element_closed_form(orc, nd):
+    # bilinear / trilinear element on a box: 2x2(x2) Gauss is exact for N_a N_b, and
+    # M_ab = rho V / 6^nd prod_d (2 if a and b share coordinate d else 1)
+    lo, hi = (0.0,) * nd, (2.0, 3.0, 0.5)[:nd]
+    kind = orc.QUAD4 if nd == 2 else orc.HEX8
+    X, conn = orc.structured_mesh(kind, lo, hi, (2,) * nd)
+    rho, V = 1.7, float(np.prod(hi))
+    for nf in (1, nd):
+        st = orc.assemble_mass_storage(kind, X, conn, nf, rho)
+        ndof = conn.shape[1] * nf
+        M = st.reshape(ndof, ndof)
+        xl = X[conn[0] - 1]
+        same = (xl[:, None, :] == xl[None, :, :])
+        Mn = rho * V / 6.0 ** nd * np.prod(np.where(same, 2.0, 1.0), axis=2)
+        np.testing.assert_allclose(M, np.kron(Mn, np.eye(nf)), rtol=1e-13, atol=1e-15)
+
+
+def test_tet10_lumped_mass_closed_form(orc):
+    # integral of a quadratic tet's vertex function is -V/20, of an edge function V/5 (the degree-2 rule is exact)
+    v = np.array([[0.0, 0.0, 0.0], [2.0, 0.0, 0.0], [0.0, 3.0, 0.0], [0.0, 0.0, 1.5]])
+    X, conn = _single_tet10(v)
+    V = 1.5
+    m = orc.assemble_lumped_mass(orc.TET10, X, conn, 1, 2.0)
+    np.testing.assert_allclose(m, 2.0 * V * np.array([-1 / 20] * 4 + [1 / 5] * 6), rtol=1e-13)
+
+
+@pytest.mark.parametrize("case", ["quad4", "hex8_nf3", "tet10_nf3"])
+def test_lumped_is_row_sum_and_diagonals(fec, orc, case):
+    # the reference's lumped-mass contract (test/TestAssemblers.jl:432-521): row sums of the consistent mass, total
+    # mass rho V per component; the diagonal assemblies equal the diagonals of the sparse! CSR matrices; the mass
+    # action equals CSR x v
+    if case == "quad4":
+        mesh, nf, kind = fec.StructuredMesh("quad", (0, 0), (1, 2), (5, 5)), 1, orc.QUAD4
+    elif case == "hex8_nf3":
+        mesh, nf, kind = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (4, 4, 4)), 3, orc.HEX8
+    else:
+        mesh, nf, kind = fec.kuhn_tet10_mesh(2), 3, orc.TET10
+    X, conn = mesh.coords, mesh.conn
+    vol = {"quad4": 2.0, "hex8_nf3": 1.0, "tet10_nf3": 1.0}[case]
+    rho = 3.0
+    N = nf * mesh.num_nodes
+    _, d2u = orc.update_dofs(N, np.zeros(0, np.int64))
+    rp, cv, mv = orc.mass_csr(kind, X, conn, nf, rho, d2u, N)
+    M = sp.csr_matrix((mv, cv - 1, rp - 1), shape=(N, N))
+    lm = orc.assemble_lumped_mass(kind, X, conn, nf, rho)
+    np.testing.assert_allclose(lm, np.asarray(M.sum(axis=1)).ravel(), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(lm.sum(), rho * vol * nf, rtol=1e-12)
+    np.testing.assert_allclose(orc.assemble_diagonal(kind, X, conn, nf, 0, [], None, which="mass", rho=rho),
+                               M.diagonal(), rtol=1e-13, atol=1e-15)
+    v = np.sin(np.arange(N) * 0.37)
+    np.testing.assert_allclose(orc.assemble_mass_action(kind, X, conn, nf, rho, v), M @ v, rtol=1e-12, atol=1e-14)
+    phys = orc.POISSON if nf == 1 else orc.NEOHOOKEAN
+    props = [1.0, 1.0] if nf == 1 else [1.0, 0.5]
+    U = 0.02 * np.cos(np.arange(N) * 0.91)
+    krp, kcv, kv = orc.stiffness_csr(kind, X, conn, nf, phys, props, U, d2u, N)
+    K = sp.csr_matrix((kv, kcv - 1, krp - 1), shape=(N, N))
+    np.testing.assert_allclose(orc.assemble_diagonal(kind, X, conn, nf, phys, props, U), K.diagonal(), rtol=1e-13,
+                               atol=1e-15)
