@@ -16,6 +16,12 @@ namespace fecx {
 
 constexpr int RED_THREADS = 256;
 constexpr int CH = 8;            // incidences staged per pass
+#ifndef RED_NF1_LPR
+#define RED_NF1_LPR 8            // lanes per scalar row of at most 32 blocks (8 or 16)
+#endif
+#ifndef RED_NF3_LPR
+#define RED_NF3_LPR 32           // lanes per vector row of at most 32 blocks
+#endif
 #ifndef RED_LB
 #define RED_LB 4                 // vector rows: incidences whose block loads are in flight together
 #endif
@@ -102,8 +108,8 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
     }
     __syncthreads();
     if (r >= 0) {
-      if constexpr (NF == 1) {
-        // scalar rows: 8 words per incidence, one load per lane and incidence
+      if constexpr (NF == 1 && LPR > BW) {
+        // scalar rows on more lanes than words per incidence: one load per lane and incidence
         for (int t = 0; t < cnt; ++t) {
           const uint32_t v = (uint32_t)s_v[grp][t];
           const uint32_t ebase = (v / NNPE) * (uint32_t)EW;  // element storage is limited to 2^31 words
@@ -120,7 +126,7 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
         // incidence's base word and are never written. LB = 4 keeps the kernel near 80 VGPRs (6 waves per SIMD);
         // 8 in flight needs ~106 and measured slower.
         constexpr int JN = (BW + LPR - 1) / LPR;
-        constexpr int LB = RED_LB;
+        constexpr int LB = NF == 1 ? CH : RED_LB;  // scalar rows: one word per lane and incidence, all in flight
 #pragma unroll
         for (int t0 = 0; t0 < CH; t0 += LB) {
           uint32_t idx[LB][JN];
@@ -197,18 +203,28 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
   }
 }
 
-// One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 16 for
-// scalar rows of up to 32 blocks (hex8: 27), 32 for up to 32 blocks, else 64 with two sweeps (tet10 vertex rows: 65).
+// One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 8 for
+// scalar rows of up to 32 blocks (hex8: 27, in four sweeps; 32 rows per work-group, every lane busy in the gather of
+// the 8-word scalar blocks: 0.46 ms vs 0.575 ms with 16 lanes on Poisson 128^3), 32 for vector rows of up to 32
+// blocks, else 64 with two sweeps (tet10 vertex rows: 65).
 template <int NF, int NNPE>
 static int launch_lpr(const ReduceArgs& a, int maxdeg, hipStream_t s) {
   if (a.p1 <= a.p0) return 0;
-  const int lpr = (NF == 1 && maxdeg <= 32) ? 16 : (maxdeg <= 32 ? 32 : 64);
-  if (maxdeg > 2 * lpr) return -1;
-  const int sw = maxdeg <= lpr ? 1 : 2;
+  const int lpr = (NF == 1 && maxdeg <= 32) ? RED_NF1_LPR : (maxdeg <= 32 ? RED_NF3_LPR : 64);
+  const int sw = maxdeg <= lpr ? 1 : (maxdeg <= 2 * lpr ? 2 : 4);
+  if (maxdeg > sw * lpr || (sw == 4 && lpr != 8)) return -1;
   const int64_t npw = RED_THREADS / lpr;
   int64_t grid = (a.p1 - a.p0 + npw - 1) / npw;
   grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
   const unsigned g = (unsigned)grid;
+  if constexpr (NF == 1) {
+    if (lpr == 8) {
+      if (sw == 1) reduce_rows_kernel<NF, NNPE, 8, 1><<<g, RED_THREADS, 0, s>>>(a);
+      else if (sw == 2) reduce_rows_kernel<NF, NNPE, 8, 2><<<g, RED_THREADS, 0, s>>>(a);
+      else reduce_rows_kernel<NF, NNPE, 8, 4><<<g, RED_THREADS, 0, s>>>(a);
+      return 0;
+    }
+  }
   if (lpr == 16 && sw == 1) reduce_rows_kernel<NF, NNPE, 16, 1><<<g, RED_THREADS, 0, s>>>(a);
   else if (lpr == 16) reduce_rows_kernel<NF, NNPE, 16, 2><<<g, RED_THREADS, 0, s>>>(a);
   else if (lpr == 32 && sw == 1) reduce_rows_kernel<NF, NNPE, 32, 1><<<g, RED_THREADS, 0, s>>>(a);
