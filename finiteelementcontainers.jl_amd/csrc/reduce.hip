@@ -22,6 +22,9 @@ constexpr int CH = 8;            // incidences staged per pass
 #ifndef RED_NF3_LPR
 #define RED_NF3_LPR 32           // lanes per vector row of at most 32 blocks
 #endif
+#ifndef RED_HEAVY_LPR
+#define RED_HEAVY_LPR 128        // lanes per row of 65..128 blocks (128: one slot per lane; 64: two sweeps)
+#endif
 #ifndef RED_LB
 #define RED_LB 4                 // vector rows: incidences whose block loads are in flight together
 #endif
@@ -129,6 +132,7 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
         constexpr int LB = NF == 1 ? CH : RED_LB;  // scalar rows: one word per lane and incidence, all in flight
 #pragma unroll
         for (int t0 = 0; t0 < CH; t0 += LB) {
+          if (t0 >= cnt) break;  // no incidence left in this batch (s_v[grp][0] is set whenever t0 < cnt)
           uint32_t idx[LB][JN];
 #pragma unroll
           for (int tt = 0; tt < LB; ++tt) {
@@ -206,11 +210,12 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
 // One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 8 for
 // scalar rows of up to 32 blocks (hex8: 27, in four sweeps; 32 rows per work-group, every lane busy in the gather of
 // the 8-word scalar blocks: 0.46 ms vs 0.575 ms with 16 lanes on Poisson 128^3), 32 for vector rows of up to 32
-// blocks, else 64 with two sweeps (tet10 vertex rows: 65).
+// blocks, 64 for up to 64, and 128 for up to 128 (tet10 vertex rows: 65 blocks; 128 lanes with one slot each measured
+// 7.59 ms for the tet10 80^3 reduction against 7.95 ms with 64 lanes in two sweeps, whose second sweep is lane 0 alone).
 template <int NF, int NNPE>
 static int launch_lpr(const ReduceArgs& a, int maxdeg, hipStream_t s) {
   if (a.p1 <= a.p0) return 0;
-  const int lpr = (NF == 1 && maxdeg <= 32) ? RED_NF1_LPR : (maxdeg <= 32 ? RED_NF3_LPR : 64);
+  const int lpr = (NF == 1 && maxdeg <= 32) ? RED_NF1_LPR : (maxdeg <= 32 ? RED_NF3_LPR : (maxdeg <= 64 ? 64 : RED_HEAVY_LPR));
   const int sw = maxdeg <= lpr ? 1 : (maxdeg <= 2 * lpr ? 2 : 4);
   if (maxdeg > sw * lpr || (sw == 4 && lpr != 8)) return -1;
   const int64_t npw = RED_THREADS / lpr;
@@ -229,7 +234,9 @@ static int launch_lpr(const ReduceArgs& a, int maxdeg, hipStream_t s) {
   else if (lpr == 16) reduce_rows_kernel<NF, NNPE, 16, 2><<<g, RED_THREADS, 0, s>>>(a);
   else if (lpr == 32 && sw == 1) reduce_rows_kernel<NF, NNPE, 32, 1><<<g, RED_THREADS, 0, s>>>(a);
   else if (lpr == 32) reduce_rows_kernel<NF, NNPE, 32, 2><<<g, RED_THREADS, 0, s>>>(a);
-  else reduce_rows_kernel<NF, NNPE, 64, 2><<<g, RED_THREADS, 0, s>>>(a);
+  else if (lpr == 64 && sw == 1) reduce_rows_kernel<NF, NNPE, 64, 1><<<g, RED_THREADS, 0, s>>>(a);
+  else if (lpr == 64) reduce_rows_kernel<NF, NNPE, 64, 2><<<g, RED_THREADS, 0, s>>>(a);
+  else reduce_rows_kernel<NF, NNPE, 128, 1><<<g, RED_THREADS, 0, s>>>(a);
   return 0;
 }
 
