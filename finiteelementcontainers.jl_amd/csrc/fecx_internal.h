@@ -124,6 +124,7 @@ struct ReduceArgs {
   int64_t p0, p1;           // processing positions [p0, p1) of this launch
   int64_t nlight;           // positions [0, nlight) hold the light rows (at most 8 incidences and 32 blocks)
   int maxdeg_light;         // largest row degree among them
+  int onepass;              // every row of this launch has at most CH incidences (one pass, no pass-count vote)
   const int32_t* norder;
   const int4* rowinfo;      // per processing position: {node, voff[node], bptr[rank] or 0, ninc | degree << 16}
   int nnpe, vstride, maxdeg;
