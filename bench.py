@@ -31,6 +31,9 @@ SIZES = {"poisson_hex8": (128, 56), "neohookean_hex8": (96, 32), "j2_hex8": (64,
 # SURVEY algorithmic bytes per element for residual + tangent (int32 conn, FP32 coords, FP64 u, FP64 outputs written
 # once, int32 COO->CSR map of NDOF^2 per element, FP32 state)
 SURVEY_BYTES = {"poisson_hex8": 534.4, "neohookean_hex8": 4362.2, "j2_hex8": 4821.4, "neohookean_tet10": 6502.3}
+# the same model for the residual alone and for the matrix-free action (FP64 v read as well)
+RESIDUAL_BYTES = {"poisson_hex8": 60.7, "neohookean_hex8": 93.9, "j2_hex8": 542.9, "neohookean_tet10": 121.5}
+MF_ACTION_BYTES = {"poisson_hex8": 68.9, "neohookean_hex8": 118.7, "j2_hex8": 568.0, "neohookean_tet10": 154.1}
 
 
 def log(msg):
@@ -159,7 +162,9 @@ def surrogate_leg(fec, torch, name, wl, asm_plain, R_plain, K_plain, steps):
         "batched_update_ms_2M": {"plain": t_plain, "surrogate": t_surr, "network_only": t_net,
                                  "surrogate_fallback_fraction": fb_b / nb},
         "throughput_ratio_vs_plain": t_plain / t_surr,
-        "mfma": {"achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS},
+        "mfma": {"achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
+                 "measured_gemm_peak": gemm_peak_tflops(torch),
+                 "in_material_kernel": MLP_FLOPS[name] * nb / (t_surr * 1e-3) / 1e12},
         "assembly_ms_per_step": {"plain_step_time_above": None, "surrogate": ms},
         "assembly_fallback_fraction": fb_a / max(np_a, 1),
         "assembly_rel_diff_vs_plain": {"residual": rel(R, R_plain), "jacobian": rel(K, K_plain)},
@@ -167,18 +172,65 @@ def surrogate_leg(fec, torch, name, wl, asm_plain, R_plain, K_plain, steps):
     return out
 
 
+def operator_timings(fec, torch, name, wl, asm, p, Uu):
+    """The other entry points on the same workload (HIP events around whole calls, DOF plumbing included): residual
+    alone, matrix-free action, assembled action; elements/s and GB/s against the algorithmic bytes per element."""
+    Vu = torch.cos(torch.arange(Uu.numel(), dtype=torch.float64, device=Uu.device))
+    ne = wl.num_elements
+    ops = {"residual": (lambda: fec.assemble_vector_(asm, fec.residual, Uu, p), RESIDUAL_BYTES[name]),
+           "matrix_free_action": (lambda: fec.assemble_matrix_free_action_(asm, fec.stiffness_action, Uu, Vu, p),
+                                  MF_ACTION_BYTES[name]),
+           "assembled_action": (lambda: fec.assemble_matrix_action_(asm, fec.stiffness, Uu, Vu, p),
+                                MF_ACTION_BYTES[name])}
+    out = {}
+    for k, (f, b) in ops.items():
+        ms = cuda_ms(torch, f, reps=10)
+        fec.reset_profile(asm)
+        fec.set_profiling(asm, True)
+        for _ in range(5):
+            f()
+        fec.set_profiling(asm, False)
+        kern = {kk: v[1] / v[0] for kk, v in fec.profile(asm).items()}
+        out[k] = {"ms": ms, "elements_per_s": ne / (ms * 1e-3), "bytes_per_element": b,
+                  "gbs": b * ne / (ms * 1e-3) / 1e9, "kernel_ms": kern}
+    log("[bench] operators: " + ", ".join(
+        f"{k} {v['ms']:.3f} ms ({v['gbs']:.0f} GB/s; " + " ".join(f"{kk} {t:.3f}" for kk, t in v["kernel_ms"].items())
+        + ")" for k, v in out.items()))
+    return out
+
+
+def gemm_peak_tflops(torch, n=8192, reps=10):
+    """Measured BF16 dense GEMM throughput (torch.matmul on contiguous n x n operands): the attainable MFMA figure
+    next to the 2.5 PFLOP/s spec."""
+    a = torch.randn(n, n, device="cuda", dtype=torch.bfloat16)
+    b = torch.randn(n, n, device="cuda", dtype=torch.bfloat16)
+    ms = cuda_ms(torch, lambda: torch.matmul(a, b), reps)
+    return 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+
+
 def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def oracle_step(orc, wl, d2u, nu):
+def oracle_step(orc, wl, pat, nu, split=None):
+    """One residual + CSR Jacobian on the CPU with the reference's algorithm: the residual loop, the element stiffness
+    storage, then sparse! over the precomputed pattern (built once, as update_dofs! does). `split`, if given, receives
+    the three times in seconds."""
+    Is, Js, slots = pat
     m = wl.mesh
     kind = {"QUAD4": orc.QUAD4, "HEX8": orc.HEX8, "TET10": orc.TET10}[m.element_type]
     props = wl.physics.props()
     st = np.zeros(m.num_elements * 8 * 7) if wl.physics.kind == orc.J2 else None
+    t0 = time.perf_counter()
     R = orc.assemble_residual(kind, m.coords, m.conn, wl.nf, wl.physics.kind, props, wl.U, st,
                               None if st is None else np.zeros_like(st))
-    rp, cv, vals = orc.stiffness_csr(kind, m.coords, m.conn, wl.nf, wl.physics.kind, props, wl.U, d2u, nu, st)
+    t1 = time.perf_counter()
+    storage = orc.assemble_stiffness_storage(kind, m.coords, m.conn, wl.nf, wl.physics.kind, props, wl.U, st)
+    t2 = time.perf_counter()
+    _, _, vals, _ = orc.sparse_csr(nu, Is, Js, orc.gather_slots(slots, storage))
+    t3 = time.perf_counter()
+    if split is not None:
+        split.append((t1 - t0, t2 - t1, t3 - t2))
     return R, vals
 
 
@@ -195,14 +247,15 @@ def sample_checks(fec, orc, name, n_sample, threads):
     unknown, d2u = orc.update_dofs(N, np.zeros(0, np.int64))
     orc.set_threads(threads)
     Is, Js, slots, perm = orc.pattern(wl.nf, wl.mesh.conn, d2u)  # setup (update_dofs!), not timed
-    oracle_step(orc, wl, d2u, len(unknown))  # warm-up
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        R, vals = oracle_step(orc, wl, d2u, len(unknown))
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return {"R": rel(R_gpu, R), "K": rel(K_gpu, vals)}, wl.mesh.num_elements, t
+    pat = (Is, Js, slots)
+    oracle_step(orc, wl, pat, len(unknown))  # warm-up
+    split = []
+    for _ in range(5):
+        R, vals = oracle_step(orc, wl, pat, len(unknown), split)
+    split = np.array(split)
+    t = float(np.median(split.sum(axis=1)))
+    parts = {k: float(np.median(split[:, i])) * 1e3 for i, k in enumerate(("residual", "element_stiffness", "sparse_csr"))}
+    return {"R": rel(R_gpu, R), "K": rel(K_gpu, vals)}, wl.mesh.num_elements, t, parts
 
 
 def main():
@@ -300,6 +353,7 @@ def main():
     if name in SURVEY_BYTES:
         eff = SURVEY_BYTES[name] * wl.num_elements / dt / 1e9
         out["step_roofline"] = {"bytes_per_element": SURVEY_BYTES[name], "achieved": eff, "frac": eff / HBM_PEAK_GBS}
+        out["operators"] = operator_timings(fec, torch, name, wl, asm, p, Uu)
 
     if name in MLP_FLOPS and not args.no_surrogate:
         R_plain = fec.residual(asm).cpu().numpy().copy()
@@ -310,13 +364,15 @@ def main():
     if not args.no_check:
         orc = ge.load_oracle()
         threads = min(16, os.cpu_count() or 1)
-        errs, ne_s, t_cpu = sample_checks(fec, orc, name, n_sample, threads)
+        errs, ne_s, t_cpu, parts = sample_checks(fec, orc, name, n_sample, threads)
         out["rel_err"] = {"residual": errs["R"], "jacobian": errs["K"],
                           "sample": f"{name} n={n_sample} ({ne_s} elements) vs FP64 oracle"}
         if not args.no_cpu_baseline:
             out["cpu_baseline"] = {"value": ne_s / t_cpu, "unit": "elements/s", "cores": threads, "kind": "port",
                                    "sample": f"{name} n={n_sample}: {ne_s} elements, residual + element stiffness + "
-                                             f"sparse! COO->CSR, median of 3 after 1 warm-up"}
+                                             f"sparse! COO->CSR over the pattern built at setup, median of 5 after "
+                                             f"1 warm-up (C++ restatement of the reference algorithm, OpenMP)",
+                                   "split_ms": parts}
     print(json.dumps(out), flush=True)
 
 

@@ -263,8 +263,49 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
   return rc;
 }
 
+// Vector-only reduction (residual, actions, diagonals: no CSR values): one thread per row sums the row's incident
+// element rows into R in ascending incidence order -- the order of the matrix kernel's residual sum, so the two give
+// the same bits. Incidences go in batches of 8: indices first, then all their loads, then the FP64 adds.
+template <int NF>
+__global__ __launch_bounds__(256) void reduce_vector_kernel(ReduceArgs a) {
+  const int64_t pos = a.p0 + xcd_chunk(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (pos >= a.p1) return;
+  const int4 ri = a.rowinfo[pos];
+  const int64_t n = ri.x;
+  const int32_t c0 = ri.y, ninc = ri.w & 0xFFFF;
+  double acc[NF];
+#pragma unroll
+  for (int i = 0; i < NF; ++i) acc[i] = 0.0;
+  for (int t0 = 0; t0 < ninc; t0 += 8) {
+    int32_t v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = t0 + t < ninc ? a.vent[c0 + t0 + t] : a.vent[c0];
+    float r[8][NF];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int i = 0; i < NF; ++i) r[t][i] = a.Rel[(int64_t)v[t] * NF + i];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      if (t0 + t < ninc)
+#pragma unroll
+        for (int i = 0; i < NF; ++i) acc[i] += (double)r[t][i];
+  }
+#pragma unroll
+  for (int i = 0; i < NF; ++i) a.R[n * NF + i] = acc[i];
+}
+
 int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s) {
   if (a.p1 <= a.p0) return 0;
+  if (!a.vals) {
+    if (!a.R) return 0;
+    int64_t grid = (a.p1 - a.p0 + 255) / 256;
+    grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
+    if (nf == 1) reduce_vector_kernel<1><<<(unsigned)grid, 256, 0, s>>>(a);
+    else if (nf == 3) reduce_vector_kernel<3><<<(unsigned)grid, 256, 0, s>>>(a);
+    else return -1;
+    return 0;
+  }
   if (nf == 1) {
     if (a.nnpe == 4) return launch_buckets<1, 4>(a, s);
     if (a.nnpe == 8) return launch_buckets<1, 8>(a, s);
