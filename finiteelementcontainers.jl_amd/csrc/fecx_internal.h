@@ -125,6 +125,8 @@ struct ReduceArgs {
   int64_t nlight;           // positions [0, nlight) hold the light rows (at most 8 incidences and 32 blocks)
   int maxdeg_light;         // largest row degree among them
   int onepass;              // every row of this launch has at most CH incidences (one pass, no pass-count vote)
+  int64_t npad;             // positions [0, npad) hold padded rows: incidence slots vent[8 p .. 8 p + 7]
+  int padded;               // this launch reads the padded slots directly (every row of it is padded)
   const int32_t* norder;
   const int4* rowinfo;      // per processing position: {node, voff[node], bptr[rank] or 0, ninc | degree << 16}
   int nnpe, vstride, maxdeg;

@@ -102,8 +102,19 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
   for (int pass = 0; pass < maxpass; ++pass) {
     const int base = pass * CH;
     const int cnt = min(CH, ninc - base);  // may be <= 0 for this node
-    if (lane < cnt) {
-      const int32_t c = c0 + base + lane;
+    // padded light rows (scalar kernels only): the 8 slots at 8 pos are loaded without waiting for the row descriptor
+    // (unused slots hold a valid incidence and are never summed: every later loop stops at cnt). The vector-row
+    // kernels keep the plain form, which holds their register count below the next occupancy step.
+    bool stage;
+    int64_t c;
+    if constexpr (NF == 1) {
+      stage = a.padded ? (valid && lane < CH) : (lane < cnt);
+      c = a.padded ? pos * CH + lane : (int64_t)c0 + base + lane;
+    } else {
+      stage = lane < cnt;
+      c = c0 + base + lane;
+    }
+    if (stage) {
       const int32_t v = a.vent[c];
       s_v[grp][lane] = v;
       const uint32_t* sl = reinterpret_cast<const uint32_t*>(a.vslot) + (int64_t)c * VW;
@@ -258,6 +269,9 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
   static_assert(CH >= 8, "light rows have at most 8 incidences and must take one pass");
   lo.onepass = NNPE == 8 ? 1 : 0;
   hi.onepass = 0;
+  // direct slot addressing measured -1 % (scalar rows) and +0.8 % (vector rows) on the matrix reduction
+  lo.padded = (NF == 1 && a.npad >= lo.p1) ? 1 : 0;
+  hi.padded = 0;
   int rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
   if (rc == 0) rc = launch_lpr<NF, NNPE>(hi, a.maxdeg, s);
   return rc;
@@ -276,10 +290,12 @@ __global__ __launch_bounds__(256) void reduce_vector_kernel(ReduceArgs a) {
   double acc[NF];
 #pragma unroll
   for (int i = 0; i < NF; ++i) acc[i] = 0.0;
-  for (int t0 = 0; t0 < ninc; t0 += 8) {
+  if (pos < a.npad) {
+    // padded light row: its 8 slots sit at 8 pos, so their loads do not wait for the descriptor (slots past ninc hold a
+    // valid incidence and are not summed)
     int32_t v[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) v[t] = t0 + t < ninc ? a.vent[c0 + t0 + t] : a.vent[c0];
+    for (int t = 0; t < 8; ++t) v[t] = a.vent[pos * 8 + t];
     float r[8][NF];
 #pragma unroll
     for (int t = 0; t < 8; ++t)
@@ -287,9 +303,25 @@ __global__ __launch_bounds__(256) void reduce_vector_kernel(ReduceArgs a) {
       for (int i = 0; i < NF; ++i) r[t][i] = a.Rel[(int64_t)v[t] * NF + i];
 #pragma unroll
     for (int t = 0; t < 8; ++t)
-      if (t0 + t < ninc)
+      if (t < ninc)
 #pragma unroll
         for (int i = 0; i < NF; ++i) acc[i] += (double)r[t][i];
+  } else {
+    for (int t0 = 0; t0 < ninc; t0 += 8) {
+      int32_t v[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = t0 + t < ninc ? a.vent[c0 + t0 + t] : a.vent[c0];
+      float r[8][NF];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int i = 0; i < NF; ++i) r[t][i] = a.Rel[(int64_t)v[t] * NF + i];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (t0 + t < ninc)
+#pragma unroll
+          for (int i = 0; i < NF; ++i) acc[i] += (double)r[t][i];
+    }
   }
 #pragma unroll
   for (int i = 0; i < NF; ++i) a.R[n * NF + i] = acc[i];
