@@ -60,14 +60,13 @@ def kernel_bytes(wl, asm, with_r=True, with_k=True):
     nd = wl.mesh.coords.shape[1]
     ndof = nnpe * nf
     npairs = nnpe * (nnpe + 1) // 2
-    nfree = len(asm.dof.unknown_dofs) // nf
     vstride = (nnpe + 3) // 4 * 4
     state = 2 * 4 * 7 * 8 * ne if wl.physics.kind == 3 else 0
     kel = 4 * npairs * nf * nf * ne if with_k else 0
     rel_ = 4 * ndof * ne if with_r else 0
     return {
         "element": 4 * nnpe * ne + 2 * 4 * nd * nn + 8 * nf * nn + rel_ + kel + state,
-        "reduce": 8 * (nn + 1) + 8 * (nfree + 1) + 4 * nn + (4 + vstride) * ne * nnpe + kel + rel_
+        "reduce": 16 * nn + (4 + vstride) * ne * nnpe + kel + rel_
         + (8 * asm.nnz if with_k else 0) + (8 * nf * nn if with_r else 0),
     }
 

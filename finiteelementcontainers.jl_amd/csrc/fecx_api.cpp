@@ -77,20 +77,14 @@ fecx::ElementArgs element_args(fecx_ctx* ctx, const double* U, const double* V) 
 
 fecx::ReduceArgs reduce_args(fecx_ctx* ctx) {
   fecx::ReduceArgs r{};
-  r.nn = ctx->nn;
   r.p0 = 0;
   r.p1 = ctx->nn;
-  r.norder = get<int32_t>(ctx, "norder");
   r.rowinfo = get<int4>(ctx, "rowinfo");
   r.nnpe = ctx->nnpe;
-  r.vstride = ctx->hp.vstride;
   r.maxdeg = ctx->hp.maxdeg;
   r.nlight = ctx->hp.nlight;
   r.npad = ctx->nnpe == 8 && ctx->nf == 1 ? ctx->hp.nlight : 0;  // padded light rows (see fecx_build_pattern)
   r.maxdeg_light = ctx->hp.maxdeg_light;
-  r.free_rank = get<int32_t>(ctx, "free_rank");
-  r.bptr = get<int64_t>(ctx, "bptr");
-  r.voff = get<int32_t>(ctx, "voff");
   r.vent = get<int32_t>(ctx, "vent");
   r.vslot = get<uint8_t>(ctx, "vslot");
   r.kel_off = get<int16_t>(ctx, "kel_off");
@@ -337,7 +331,6 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
     upload(ctx, "conn", conn_s);
     upload(ctx, "eperm", hp.eperm);
     upload(ctx, "epos", hp.epos);
-    upload(ctx, "norder", hp.norder);
     {  // reduction row descriptors (16 bytes, one load instead of a chain of dependent ones) and incidence lists.
        // Scalar hex8: lists in processing order, light rows (at most 8 incidences; 8 for every interior node) in 8
        // slots each, row p at 8 p, so the reduction loads a light row's list together with its descriptor instead of
@@ -382,10 +375,8 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
       upload(ctx, "vent", pordered ? pv : hp.vent);
       upload(ctx, "vslot", pordered ? ps : hp.vslot);
     }
-    upload(ctx, "free_rank", hp.free_rank);
     upload(ctx, "bptr", hp.bptr);
     upload(ctx, "bcol", hp.bcol);
-    upload(ctx, "voff", hp.voff);
     {  // staged row word j = (b, c) of local node a -> word inside the symmetric element storage (K_ba transposed)
       const int nnpe = ctx->nnpe, bw = nnpe * nf * nf;
       std::vector<int16_t> off((size_t)nnpe * bw);

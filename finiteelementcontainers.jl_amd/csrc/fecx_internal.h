@@ -120,19 +120,14 @@ int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t
 int launch_mass(int kind, int nf, int what, const ElementArgs& a, float rho, hipStream_t s);
 
 struct ReduceArgs {
-  int64_t nn;
   int64_t p0, p1;           // processing positions [p0, p1) of this launch
   int64_t nlight;           // positions [0, nlight) hold the light rows (at most 8 incidences and 32 blocks)
   int maxdeg_light;         // largest row degree among them
   int onepass;              // every row of this launch has at most CH incidences (one pass, no pass-count vote)
   int64_t npad;             // positions [0, npad) hold padded rows: incidence slots vent[8 p .. 8 p + 7]
   int padded;               // this launch reads the padded slots directly (every row of it is padded)
-  const int32_t* norder;
-  const int4* rowinfo;      // per processing position: {node, voff[node], bptr[rank] or 0, ninc | degree << 16}
-  int nnpe, vstride, maxdeg;
-  const int32_t* free_rank;
-  const int64_t* bptr;
-  const int32_t* voff;
+  const int4* rowinfo;      // per processing position: {node, first incidence slot, first CSR block or 0, ninc | degree << 16}
+  int nnpe, maxdeg;
   const int32_t* vent;
   const uint8_t* vslot;
   const int16_t* kel_off;   // nnpe x (nnpe*nf*nf): word of (local node a, staged word j) in an element's storage
