@@ -169,11 +169,11 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
     require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
     HIPCHK(hipGetLastError());
   }
+  int hflag = 0;
+  HIPCHK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   prof.finish();
   if (surrogate) fecx::surrogate_qp_finish(ctx);
-  int hflag = 0;
-  HIPCHK(hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost));
   require(hflag == 0, FECX_ERR_NUMERIC, "element kernel met a non-positive Jacobian determinant or a NaN");
 }
 
@@ -198,10 +198,10 @@ void assemble_mass(fecx_ctx* ctx, int what, double density, const double* V, dou
   prof.run("reduce", ctx->stream, [&] { rc = fecx::launch_reduce(ctx->nf, ra, ctx->stream); });
   require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
   HIPCHK(hipGetLastError());
+  int hflag = 0;
+  HIPCHK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   prof.finish();
-  int hflag = 0;
-  HIPCHK(hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost));
   require(hflag == 0, FECX_ERR_NUMERIC, "mass kernel met a non-positive Jacobian determinant");
 }
 
@@ -585,7 +585,6 @@ int fecx_update_field_unknowns(fecx_ctx* ctx, const double* Uu, double* U) {
     fecx::launch_scatter_unknowns((int64_t)ctx->unknown_dofs.size(), get<int64_t>(ctx, "unknown_idx"), Uu, U,
                                   ctx->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(ctx->stream));
   });
 }
 
@@ -596,7 +595,6 @@ int fecx_update_field_dirichlet(fecx_ctx* ctx, const double* vals, double* U) {
     fecx::launch_scatter_unknowns((int64_t)ctx->dirichlet.size(), get<int64_t>(ctx, "dirichlet_idx"), vals, U,
                                   ctx->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(ctx->stream));
   });
 }
 
@@ -607,7 +605,6 @@ int fecx_extract_field_unknowns(fecx_ctx* ctx, const double* U, double* Uu) {
     fecx::launch_gather_unknowns((int64_t)ctx->unknown_dofs.size(), get<int64_t>(ctx, "unknown_idx"), U, Uu,
                                  ctx->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(ctx->stream));
   });
 }
 

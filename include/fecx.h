@@ -16,10 +16,12 @@
  *    live on the context's device, e.g. a Julia ROCArray or a torch tensor; no copies.
  *  - Indices crossing the boundary are 1-based int64 like the reference, unless stated.
  *  - Nodal fields are NF x NN column-major (dof = NF*(node-1)+d, src/Fields.jl:36-40).
- *  - Calls on one context are not re-entrant. Assembly calls are enqueued on the
- *    context's stream (fecx_set_stream) and the call returns after the stream is
- *    synchronised, so results can be read at once (the reference's callers read them
- *    immediately, src/Solvers.jl:128-153).
+ *  - Calls on one context are not re-entrant. Device work is enqueued on the context's
+ *    stream (fecx_set_stream). Assembly calls return after the stream is synchronised,
+ *    so results can be read at once (the reference's callers read them immediately,
+ *    src/Solvers.jl:128-153) and a folded element is reported by that call. The DOF
+ *    plumbing calls (fecx_update_field_*, fecx_extract_field_unknowns) only enqueue:
+ *    work ordered on the same stream sees their results.
  */
 #ifndef FECX_H
 #define FECX_H
