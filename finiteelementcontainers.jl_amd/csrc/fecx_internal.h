@@ -85,6 +85,13 @@ void build_host_pattern(HostPattern& hp, int64_t nn, int64_t ne, int nnpe, const
                         const std::vector<char>& node_free, const PairTable& pt, int nd, const double* coords);
 
 // ------------------------------------------------------------------ kernels
+// Work-groups are dealt to the 8 XCDs round-robin; remap block index wg so that each XCD walks a contiguous range of
+// work (rows or elements), which keeps the data shared by neighbouring rows or elements in that XCD's L2.
+__device__ __forceinline__ int64_t xcd_chunk(int64_t wg, int64_t nwg) {
+  const int64_t per = nwg / 8;  // nwg is a multiple of 8
+  return (wg % 8) * per + wg / 8;
+}
+
 struct MatParams {
   int phys;
   int source_kind;

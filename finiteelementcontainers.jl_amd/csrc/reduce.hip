@@ -29,13 +29,6 @@ constexpr int CH = 8;            // incidences staged per pass
 #define RED_LB 4                 // vector rows: incidences whose block loads are in flight together
 #endif
 
-// Work-groups are dealt to the 8 XCDs round-robin; remap so that each XCD walks a contiguous range of nodes,
-// which keeps the element blocks shared by neighbouring nodes in that XCD's L2.
-__device__ __forceinline__ int64_t xcd_chunk(int64_t wg, int64_t nwg) {
-  const int64_t per = nwg / 8;  // nwg is a multiple of 8
-  return (wg % 8) * per + wg / 8;
-}
-
 // Position of byte == s in the packed slot words (at most one match: a node appears once per element), or -1.
 template <int VW>
 __device__ __forceinline__ int match_slot(const uint32_t* w, int s) {
