@@ -173,35 +173,61 @@ __device__ __forceinline__ void gelu_pack(const f32x16* acc, bf16x8* hb) {
       for (int i = 0; i < 8; ++i) hb[2 * mt + s][i] = (__bf16)gelu(acc[mt][8 * s + i]);
 }
 
+// Z = W_l H + b for one hidden layer, with the bias, GELU and BF16 packing applied one 32-feature D tile at a time
+// straight into the next layer's B operands: one tile (16 registers) is live instead of P[l+1] / 2, so the inference
+// kernels fit more wavefronts per SIMD and one wavefront's activations overlap another's MFMAs. Same arithmetic, in the
+// same order, as mlp_layer followed by gelu_pack.
+template <class A, int L>
+__device__ __forceinline__ void mlp_layer_gelu(const MlpDev& m, const bf16x8* hb, bf16x8* hn, int lane) {
+  constexpr int IN = A::P[L], OUT = A::P[L + 1];
+  const bf16x8* w = m.wf + Offs<A>::wf(L);
+  const float* b = m.bias + Offs<A>::bias(L);
+  const int h = lane >> 5;
+#pragma unroll
+  for (int mt = 0; mt < OUT / 32; ++mt) {
+    f32x16 c;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) c[j] = b[dreg_feature(mt, h, j)];
+#pragma unroll
+    for (int kb = 0; kb < IN / 16; ++kb)
+      c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[(mt * (IN / 16) + kb) * 64 + lane], hb[kb], c, 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) hn[2 * mt + s][i] = (__bf16)gelu(c[8 * s + i]);
+  }
+}
+
 // Forward pass of one wavefront (32 samples). xin: this lane's sample features, already normalized (the first D[0]
 // of them); on return y holds all P[NL] normalized outputs of this lane's sample (both lane halves agree).
 template <class A>
 __device__ __forceinline__ void mlp_forward_wave(const MlpDev& m, const float* xin, float* y, int lane) {
-  constexpr int MAXW = max_hidden<A>();
+  static_assert(A::NL == 3 || A::NL == 4, "three or four layers");
+  static_assert(A::P[0] == 16 && A::P[A::NL] == 32, "one input block and one output tile");
   const int h = lane >> 5;
-  bf16x8 hb[MAXW / 16];
-  f32x16 acc[MAXW / 32];
+  bf16x8 h0[1];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {  // lane half h takes features 8 h .. 8 h + 7 (compile-time indices only)
     const float v0 = i < A::D[0] ? xin[i] : 0.0f;
     const float v1 = 8 + i < A::D[0] ? xin[(8 + i) % A::D[0]] : 0.0f;
-    hb[0][i] = (__bf16)(h ? v1 : v0);
+    h0[0][i] = (__bf16)(h ? v1 : v0);
   }
-  mlp_layer<A, 0>(m, hb, acc, lane);
-  gelu_pack<A::P[1]>(acc, hb);
-  mlp_layer<A, 1>(m, hb, acc, lane);
-  if constexpr (A::NL >= 3) {
-    gelu_pack<A::P[2]>(acc, hb);
-    mlp_layer<A, 2>(m, hb, acc, lane);
+  bf16x8 h1[A::P[1] / 16];
+  mlp_layer_gelu<A, 0>(m, h0, h1, lane);
+  f32x16 out[1];
+  if constexpr (A::NL == 3) {
+    bf16x8 h2[A::P[2] / 16];
+    mlp_layer_gelu<A, 1>(m, h1, h2, lane);
+    mlp_layer<A, 2>(m, h2, out, lane);
+  } else {
+    bf16x8 h2[A::P[2] / 16], h3[A::P[3] / 16];
+    mlp_layer_gelu<A, 1>(m, h1, h2, lane);
+    mlp_layer_gelu<A, 2>(m, h2, h3, lane);
+    mlp_layer<A, 3>(m, h3, out, lane);
   }
-  if constexpr (A::NL >= 4) {
-    gelu_pack<A::P[3]>(acc, hb);
-    mlp_layer<A, 3>(m, hb, acc, lane);
-  }
-  static_assert(A::P[A::NL] == 32, "one output tile");
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    const float mine = acc[0][j], other = __shfl_xor(mine, 32);
+    const float mine = out[0][j], other = __shfl_xor(mine, 32);
     y[dreg_feature(0, 0, j)] = h ? other : mine;
     y[dreg_feature(0, 1, j)] = h ? mine : other;
   }
