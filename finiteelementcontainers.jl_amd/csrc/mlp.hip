@@ -272,12 +272,13 @@ __global__ void mlp_update_kernel(UpdateArgs a) {
   if (a.repack) repack_param<ARCH>(a.pk, p, w);
 }
 
+// Sum of the n loss partials by one wavefront in a fixed order (lane l takes partials l, l + 64, ..., then a fixed
+// xor tree across lanes), so every run gives the same bits.
 __global__ void loss_sum_kernel(const double* part, int64_t n, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0.0;
-    for (int64_t i = 0; i < n; ++i) s += part[i];
-    out[0] = s;
-  }
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 64) s += part[i];
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+  if (threadIdx.x == 0) out[0] = s;
 }
 
 __global__ void gather_rows_kernel(int64_t n, int w, const int32_t* idx, const float* src, float* dst) {
