@@ -28,9 +28,9 @@ HBM_PEAK_GBS = 8000.0
 # per-config full sizes and the bounded sample used for parity and the CPU baseline
 SIZES = {"poisson_hex8": (128, 56), "neohookean_hex8": (96, 32), "j2_hex8": (64, 28), "neohookean_tet10": (80, 14),
          "poisson_quad4": (64, 64)}
-# SURVEY algorithmic bytes per element for residual + tangent (int32 conn, FP32 coords, FP64 u, FP64 outputs written
+# algorithmic (compulsory) bytes per element for residual + tangent (int32 conn, FP32 coords, FP64 u, FP64 outputs written
 # once, int32 COO->CSR map of NDOF^2 per element, FP32 state)
-SURVEY_BYTES = {"poisson_hex8": 534.4, "neohookean_hex8": 4362.2, "j2_hex8": 4821.4, "neohookean_tet10": 6502.3}
+STEP_BYTES = {"poisson_hex8": 534.4, "neohookean_hex8": 4362.2, "j2_hex8": 4821.4, "neohookean_tet10": 6502.3}
 # the same model for the residual alone and for the matrix-free action (FP64 v read as well)
 RESIDUAL_BYTES = {"poisson_hex8": 60.7, "neohookean_hex8": 93.9, "j2_hex8": 542.9, "neohookean_tet10": 121.5}
 MF_ACTION_BYTES = {"poisson_hex8": 68.9, "neohookean_hex8": 118.7, "j2_hex8": 568.0, "neohookean_tet10": 154.1}
@@ -283,7 +283,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU assembly is a design in "
-              "DESIGN.md, not a measured path", file=sys.stderr)
+              "designed (element partition, halo exchange of partial rows) but not a measured path", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -349,9 +349,9 @@ def main():
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic},
         "kernels": kernels,
     }
-    if name in SURVEY_BYTES:
-        eff = SURVEY_BYTES[name] * wl.num_elements / dt / 1e9
-        out["step_roofline"] = {"bytes_per_element": SURVEY_BYTES[name], "achieved": eff, "frac": eff / HBM_PEAK_GBS}
+    if name in STEP_BYTES:
+        eff = STEP_BYTES[name] * wl.num_elements / dt / 1e9
+        out["step_roofline"] = {"bytes_per_element": STEP_BYTES[name], "achieved": eff, "frac": eff / HBM_PEAK_GBS}
         out["operators"] = operator_timings(fec, torch, name, wl, asm, p, Uu)
 
     if name in MLP_FLOPS and not args.no_surrogate:

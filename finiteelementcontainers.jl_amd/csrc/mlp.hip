@@ -3,7 +3,7 @@
 //
 // The reference has no surrogate and no batched material pass; its materials are evaluated inside the element loop
 // (test/mechanics/TestMechanicsCommon.jl:140-222). The plain models here are the same FP32 material functions the
-// element kernels use (fem_device.h), so a surrogate is judged against exactly what the plain assembly computes.
+// element kernels use (fem_device.h), so a surrogate is checked against exactly what the plain assembly computes.
 #include "fem_device.h"
 #include "mlp_device.h"
 #include "surrogate.h"

@@ -214,7 +214,7 @@ int fecx_vector_axpy(fecx_ctx* ctx, int64_t n, double a, const double* x, double
  * layers runs batched over quadrature points on BF16 MFMA with FP32 accumulation. It is trained on
  * the GPU by this library's own forward/backward kernels, on samples the plain material model
  * generates. The plain model stays the fallback wherever the surrogate's output fails a residual
- * check of the material equations, and it is the judge of the surrogate's error.
+ * check of the material equations, and it is the reference for the surrogate's error.
  *   FECX_MLP_J2  13 -> 128 -> 128 -> 128 -> 13. Raw input per sample (16 floats): H (9,
  *                row-major displacement gradient), state (7: plastic strain in the order
  *                11,21,31,22,32,33, then eqps). Raw output (13): stress (6, same order), plastic

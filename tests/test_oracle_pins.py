@@ -331,7 +331,7 @@ def test_mechanics_patch_and_tangent_fd(fec, orc, phys):
         np.testing.assert_allclose(KV, Kd @ v, rtol=1e-12, atol=1e-12 * np.abs(KV).max())
 
 
-def test_c1_poisson_cg_analytic(fec, orc):
+def test_poisson_quad4_64_cg_analytic(fec, orc):
     wl = fec.workloads.poisson_quad4(64)
     mesh = wl.mesh
     N = mesh.num_nodes

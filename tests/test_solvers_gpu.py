@@ -1,9 +1,9 @@
 """GPU linear algebra on the assembled CSR Jacobian and the Newton loop (src/Solvers.jl:95-220): SpMV and CG against
-scipy on the same CSR, the C1 Poisson problem solved end to end on the GPU against its analytic solution, and Newton
+scipy on the same CSR, the 64 x 64 quad4 Poisson problem solved end to end on the GPU against its analytic solution, and Newton
 on J2 load steps.
 
 Tolerances: SpMV relative 1e-14 (FP64, different summation order); CG relative error vs a direct solve <= 1e-7
-(stopping rule ||r|| <= 1e-10 ||b||); C1 max nodal error vs the analytic solution <= 1e-3 (Q1 discretisation error at
+(stopping rule ||r|| <= 1e-10 ||b||); 64 x 64 Poisson max nodal error vs the analytic solution <= 1e-3 (Q1 discretisation error at
 h = 1/64 is ~2e-4)."""
 import numpy as np
 import pytest
@@ -48,12 +48,12 @@ def test_spmv_and_cg_match_scipy(fec, gpu):
     assert 0 < it < 1000 and err <= 1e-7
 
 
-def test_c1_poisson_newton_cg_on_gpu(fec, gpu):
-    """C1 (64 x 64 quad4, f = 2 pi^2 sin sin, u = 0 on the boundary) assembled and solved on the GPU: one Newton step
+def test_poisson_quad4_64_newton_cg_on_gpu(fec, gpu):
+    """Poisson on 64 x 64 quad4 (f = 2 pi^2 sin sin, u = 0 on the boundary) assembled and solved on the GPU: one Newton step
     of a linear problem, CG to 1e-12; the nodal solution against sin(pi x) sin(pi y). The residual's floor is FP32
     element rounding: about 4e-5 of the initial residual here (nodal residuals are cancelling sums of O(1) element
     contributions), so Newton stops at a relative residual of 1e-4."""
-    print("[solvers] C1 on the GPU", flush=True)
+    print("[solvers] Poisson 64x64 quad4 on the GPU", flush=True)
     wl, asm, p = _poisson(fec, 64)
     Uu = fec.create_unknowns(asm)
     ls = fec.IterativeLinearSolver(asm, "cg", fec.IterativeLinearSolverSettings(rtol=1e-12, atol=0.0))
@@ -65,7 +65,7 @@ def test_c1_poisson_newton_cg_on_gpu(fec, gpu):
     u = np.zeros(wl.mesh.num_nodes)
     u[asm.dof.unknown_dofs - 1] = Uu.cpu().numpy()
     err = np.max(np.abs(u - exact))
-    print(f"[solvers] C1 max nodal error {err:.3e}", flush=True)
+    print(f"[solvers] Poisson 64x64 max nodal error {err:.3e}", flush=True)
     assert err <= 1e-3
     assert hist[-1].converged
 
