@@ -148,7 +148,7 @@ def test_numeric_error_flag(fec, gpu):
     assert ei.value.code == -5
 
 
-@pytest.mark.parametrize("name", ["neohookean_hex8", "poisson_tet10"])
+@pytest.mark.parametrize("name", ["neohookean_hex8", "poisson_tet10", "poisson_hex8"])
 def test_pipelined_chunks_bit_identical(fec, gpu, name):
     """Pipelined matrix assembly (element and reduction pieces on two streams) gives the same bits as one piece."""
     print(f"[gpu chunks] {name}", flush=True)
