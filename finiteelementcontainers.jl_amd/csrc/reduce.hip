@@ -28,6 +28,12 @@ constexpr int CH = 8;            // incidences staged per pass
 #ifndef RED_LB
 #define RED_LB 4                 // vector rows: incidences whose block loads are in flight together
 #endif
+#ifndef RED_LB1
+#define RED_LB1 8                // the same for rows with one staged word per lane and incidence
+#endif
+#ifndef RED_HEAVY_CH
+#define RED_HEAVY_CH 24          // incidences staged per pass on 128-lane rows (tet10 interior vertices: 24)
+#endif
 
 // Position of byte == s in the packed slot words (at most one match: a node appears once per element), or -1.
 template <int VW>
@@ -43,24 +49,26 @@ __device__ __forceinline__ int match_slot(const uint32_t* w, int s) {
 }
 
 // A work-group owns 256/LPR consecutive nodes (after the XCD remap). Per pass, a node's LPR lanes
-//   1. stage up to CH incidences (element storage position, local node a, column slots, residual row) in LDS,
+//   1. stage up to CHK incidences (element storage position, local node a, column slots, residual row) in LDS:
+//      8, or 24 on the 128-lane rows (a tet10 interior vertex has 24 incident elements: one pass instead of three),
 //   2. copy the NNPE blocks K_ab (b = 0..NNPE-1) of each staged incidence from the symmetric element storage into
 //      LDS; a per-work-group table gives, for (a, word j of the staged row), the word's offset inside the element's
 //      storage with the transposition of blocks stored as K_ba folded in,
 //   3. sum in FP64, lane s owning column slot s, the staged blocks that land in slot s, in incidence order.
 // The residual sum runs over the same staged rows in the same order.
-template <int NF, int NNPE, int LPR, int SW>
+template <int NF, int NNPE, int LPR, int SW, int CHK = CH>
 __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) {
+  static_assert(CHK <= LPR && CHK % 8 == 0, "a row's lanes stage its incidences, 8 per batch");
   constexpr int NPW = RED_THREADS / LPR;  // nodes per work-group
   constexpr int NPAIRS = NNPE * (NNPE + 1) / 2;
   constexpr int EW = NPAIRS * NF * NF;    // stored words per element
   constexpr int VW = (NNPE + 3) / 4;      // vslot words per incidence
   constexpr int BW = NNPE * NF * NF;      // staged words per incidence
   __shared__ int16_t s_off[NNPE][BW];
-  __shared__ int32_t s_v[NPW][CH];
-  __shared__ uint32_t s_w[NPW][CH][VW];
-  __shared__ float s_r[NPW][CH][NF];
-  __shared__ float s_blk[NPW][CH][BW];
+  __shared__ int32_t s_v[NPW][CHK];
+  __shared__ uint32_t s_w[NPW][CHK][VW];
+  __shared__ float s_r[NPW][CHK][NF];
+  __shared__ float s_blk[NPW][CHK][BW];
   __shared__ int s_maxpass;
   for (int i = threadIdx.x; i < NNPE * BW; i += RED_THREADS) (&s_off[0][0])[i] = a.kel_off[i];
   const int64_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
@@ -71,11 +79,11 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
   const int4 ri = valid ? a.rowinfo[pos] : make_int4(0, 0, 0, 0);
   const int64_t n = ri.x;
   const int32_t c0 = ri.y, ninc = ri.w & 0xFFFF;
-  int maxpass = 1;  // light rows (at most CH incidences) take one pass
+  int maxpass = 1;  // light rows (at most CHK incidences) take one pass
   if (a.onepass) {
     __syncthreads();  // s_off
   } else {
-    const int npass = (ninc + CH - 1) / CH;
+    const int npass = (ninc + CHK - 1) / CHK;
     if (threadIdx.x == 0) s_maxpass = 0;
     __syncthreads();
     if (lane == 0) atomicMax(&s_maxpass, npass);
@@ -93,16 +101,16 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
     for (int c = 0; c < NF * NF; ++c) acc[t][c] = 0.0;
   double racc = 0.0;
   for (int pass = 0; pass < maxpass; ++pass) {
-    const int base = pass * CH;
-    const int cnt = min(CH, ninc - base);  // may be <= 0 for this node
+    const int base = pass * CHK;
+    const int cnt = min(CHK, ninc - base);  // may be <= 0 for this node
     // padded light rows (scalar kernels only): the 8 slots at 8 pos are loaded without waiting for the row descriptor
     // (unused slots hold a valid incidence and are never summed: every later loop stops at cnt). The vector-row
     // kernels keep the plain form, which holds their register count below the next occupancy step.
     bool stage;
     int64_t c;
     if constexpr (NF == 1) {
-      stage = a.padded ? (valid && lane < CH) : (lane < cnt);
-      c = a.padded ? pos * CH + lane : (int64_t)c0 + base + lane;
+      stage = (CHK == CH && a.padded) ? (valid && lane < CH) : (lane < cnt);
+      c = (CHK == CH && a.padded) ? pos * CH + lane : (int64_t)c0 + base + lane;
     } else {
       stage = lane < cnt;
       c = c0 + base + lane;
@@ -135,12 +143,13 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
       } else {
         // vector rows: incidences in batches of LB -- all word indices of a batch first, then its loads back to back
         // (their latencies overlap), then the LDS writes. Slots past the row's incidences re-read the first
-        // incidence's base word and are never written. LB = 4 keeps the kernel near 80 VGPRs (6 waves per SIMD);
-        // 8 in flight needs ~106 and measured slower.
+        // incidence's base word and are never written. With three words per lane and incidence (32 lanes), LB = 4
+        // keeps the kernel near 80 VGPRs (6 waves per SIMD); 8 in flight needs ~106 and measured slower. With one word
+        // per lane (scalar rows, 128-lane rows) 8 incidences cost 16 VGPRs and are all in flight.
         constexpr int JN = (BW + LPR - 1) / LPR;
-        constexpr int LB = NF == 1 ? CH : RED_LB;  // scalar rows: one word per lane and incidence, all in flight
+        constexpr int LB = (NF == 1 || JN == 1) ? RED_LB1 : RED_LB;
 #pragma unroll
-        for (int t0 = 0; t0 < CH; t0 += LB) {
+        for (int t0 = 0; t0 < CHK; t0 += LB) {
           if (t0 >= cnt) break;  // no incidence left in this batch (s_v[grp][0] is set whenever t0 < cnt)
           uint32_t idx[LB][JN];
 #pragma unroll
@@ -220,7 +229,8 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_rows_kernel(ReduceArgs a) 
 // scalar rows of up to 32 blocks (hex8: 27, in four sweeps; 32 rows per work-group, every lane busy in the gather of
 // the 8-word scalar blocks: 0.46 ms vs 0.575 ms with 16 lanes on Poisson 128^3), 32 for vector rows of up to 32
 // blocks, 64 for up to 64, and 128 for up to 128 (tet10 vertex rows: 65 blocks; 128 lanes with one slot each measured
-// 7.59 ms for the tet10 80^3 reduction against 7.95 ms with 64 lanes in two sweeps, whose second sweep is lane 0 alone).
+// 7.59 ms for the tet10 80^3 reduction against 7.95 ms with 64 lanes in two sweeps, whose second sweep is lane 0 alone;
+// staging all 24 incidences in one pass with 8 loads in flight per lane then took it from 7.82 to 7.21 ms).
 template <int NF, int NNPE>
 static int launch_lpr(const ReduceArgs& a, int maxdeg, hipStream_t s) {
   if (a.p1 <= a.p0) return 0;
@@ -245,7 +255,7 @@ static int launch_lpr(const ReduceArgs& a, int maxdeg, hipStream_t s) {
   else if (lpr == 32) reduce_rows_kernel<NF, NNPE, 32, 2><<<g, RED_THREADS, 0, s>>>(a);
   else if (lpr == 64 && sw == 1) reduce_rows_kernel<NF, NNPE, 64, 1><<<g, RED_THREADS, 0, s>>>(a);
   else if (lpr == 64) reduce_rows_kernel<NF, NNPE, 64, 2><<<g, RED_THREADS, 0, s>>>(a);
-  else reduce_rows_kernel<NF, NNPE, 128, 1><<<g, RED_THREADS, 0, s>>>(a);
+  else reduce_rows_kernel<NF, NNPE, 128, 1, RED_HEAVY_CH><<<g, RED_THREADS, 0, s>>>(a);
   return 0;
 }
 
