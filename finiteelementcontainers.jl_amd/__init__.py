@@ -4,7 +4,7 @@ assembled by hand-written gfx950 HIP kernels in libfecx.so (C ABI: include/fecx.
 The package directory name contains a dot, so load it by path (``__graft_entry__.load_package()``).
 """
 from ._lib import FECXError, exported_symbols, LIB_PATH  # noqa: F401
-from .mesh import StructuredMesh, TetMesh, kuhn_tet10_mesh, boundary_nodes  # noqa: F401
+from .mesh import StructuredMesh, TetMesh, fan_tet10_mesh, kuhn_tet10_mesh, boundary_nodes  # noqa: F401
 from .exodus import ExodusMesh, UnstructuredMesh, write_exodus  # noqa: F401
 from .assembler import (  # noqa: F401
     Poisson, Mechanics, NeoHookean, J2Plasticity, ThreeDimensional,

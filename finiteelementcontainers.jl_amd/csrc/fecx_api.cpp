@@ -323,7 +323,8 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
                              ctx->pairs, ctx->nd, ctx->coords.data());
     const int nf = ctx->nf;
     const auto& hp = ctx->hp;
-    require(hp.maxdeg <= 128, FECX_ERR_UNSUPPORTED, "a node couples to more than 128 nodes");
+    // column slots are bytes with 0xFF for "not a free node"; the reduction takes rows of up to 256 blocks
+    require(hp.maxdeg <= 255, FECX_ERR_UNSUPPORTED, "a node couples to more than 255 nodes");
     ctx->nnz = hp.nnzb * nf * nf;
     std::vector<int32_t> conn_s(ctx->ne * ctx->nnpe);  // connectivity in element storage order
     for (int64_t s = 0; s < ctx->ne; ++s)

@@ -60,7 +60,7 @@ __device__ __forceinline__ int match_slot(const uint32_t* w, int s) {
 // occupancy step up, and its reduction measured 1.043 against 1.084 ms on neo-Hookean hex8 96^3. The scalar kernels
 // already fit 8 waves (the tet10 one would spill under the cap).
 template <int NF, int NNPE, int LPR, int SW, int CHK = CH>
-__global__ __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(NF == 3 ? 7 : 1, 8)))
+__global__ __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(NF == 3 && SW == 1 ? 7 : 1, 8)))
 void reduce_rows_kernel(ReduceArgs a) {
   static_assert(CHK <= LPR && CHK % 8 == 0, "a row's lanes stage its incidences, 8 per batch");
   constexpr int NPW = RED_THREADS / LPR;  // nodes per work-group
@@ -240,8 +240,9 @@ static int launch_lpr(const ReduceArgs& a, int maxdeg, hipStream_t s) {
   if (a.p1 <= a.p0) return 0;
   const int lpr = (NF == 1 && maxdeg <= 32) ? RED_NF1_LPR : (maxdeg <= 32 ? RED_NF3_LPR : (maxdeg <= 64 ? 64 : RED_HEAVY_LPR));
   const int sw = maxdeg <= lpr ? 1 : (maxdeg <= 2 * lpr ? 2 : 4);
-  // more than one sweep only on the narrow scalar rows: wider rows move to the next lane count first
-  if (maxdeg > sw * lpr || (sw == 4 && lpr != 8) || (sw > 1 && lpr > 16)) return -1;
+  // more than one sweep on the narrow scalar rows, where wider rows move to the next lane count first, and on the
+  // 128-lane rows (129..256 blocks: vertex rows of unstructured tet10 meshes); wider rows are refused
+  if (maxdeg > sw * lpr || (sw == 4 && lpr != 8) || (sw > 1 && lpr > 16 && lpr != 128)) return -1;
   const int64_t npw = RED_THREADS / lpr;
   int64_t grid = (a.p1 - a.p0 + npw - 1) / npw;
   grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
@@ -258,7 +259,8 @@ static int launch_lpr(const ReduceArgs& a, int maxdeg, hipStream_t s) {
   else if (lpr == 16) reduce_rows_kernel<NF, NNPE, 16, 2><<<g, RED_THREADS, 0, s>>>(a);
   else if (lpr == 32) reduce_rows_kernel<NF, NNPE, 32, 1><<<g, RED_THREADS, 0, s>>>(a);
   else if (lpr == 64) reduce_rows_kernel<NF, NNPE, 64, 1><<<g, RED_THREADS, 0, s>>>(a);
-  else reduce_rows_kernel<NF, NNPE, 128, 1, RED_HEAVY_CH><<<g, RED_THREADS, 0, s>>>(a);
+  else if (sw == 1) reduce_rows_kernel<NF, NNPE, 128, 1, RED_HEAVY_CH><<<g, RED_THREADS, 0, s>>>(a);
+  else reduce_rows_kernel<NF, NNPE, 128, 2, RED_HEAVY_CH><<<g, RED_THREADS, 0, s>>>(a);
   return 0;
 }
 

@@ -125,6 +125,30 @@ _PERMS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
 _PERM_SIGN = [1, -1, -1, 1, 1, -1]
 
 
+def fan_tet10_mesh(m):
+    """Tetrahedra around one vertex: a ring of m points in the z = 0 plane with an apex above and one below it, 2m TET10
+    elements (Exodus TETRA10 node order). The centre vertex couples with 5m + 5 nodes: a row wider than the structured
+    meshes' (165 column blocks at m = 32)."""
+    V = [np.zeros(3), np.array([0.0, 0.0, 1.0]), np.array([0.0, 0.0, -1.0])]
+    V += [np.array([np.cos(t), np.sin(t), 0.0]) for t in 2 * np.pi * np.arange(m) / m]
+    tets = []
+    for i in range(m):
+        a, b = 3 + i, 3 + (i + 1) % m
+        tets += [(0, a, b, 1), (0, b, a, 2)]  # both positively oriented
+    mid = {}
+    conn = []
+    for t in tets:
+        row = list(t)
+        for a, b in [(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]:  # TETRA10 edge midpoints
+            key = (min(t[a], t[b]), max(t[a], t[b]))
+            if key not in mid:
+                mid[key] = len(V)
+                V.append(0.5 * (V[key[0]] + V[key[1]]))
+            row.append(mid[key])
+        conn.append(row)
+    return TetMesh(np.array(V), np.array(conn, dtype=np.int64) + 1)
+
+
 def kuhn_tet10_mesh(n, jitter=0.2, seed=0, stream=7):
     """Unit cube, n^3 cubes x 6 Kuhn tets, interior vertices jittered U[-jitter*h, jitter*h] per coordinate."""
     h = 1.0 / n

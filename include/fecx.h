@@ -103,7 +103,8 @@ int fecx_get_unknown_dofs(fecx_ctx* ctx, int64_t* n_unknowns, int64_t* unknown_d
 /* Builds the reduced CSR sparsity pattern and the COO->CSR index maps on the host and uploads
  * them. Replaces SparseMatrixPattern(dof) + _update_dofs!(pattern, ...) + the COO->CSC->CSR
  * conversion of stiffness(asm) (src/assemblers/SparsityPatterns.jl:53-117,160-231,301-329).
- * *nnz receives the number of CSR entries. */
+ * *nnz receives the number of CSR entries. A node may couple with at most 255 nodes (itself
+ * included): FECX_ERR_UNSUPPORTED otherwise. */
 int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz);
 
 /* CSR pattern (n_unknowns+1 row pointers, nnz column indices) with the given index base

@@ -166,9 +166,11 @@ def test_pipelined_chunks_bit_identical(fec, gpu, name):
         assert np.array_equal(R, out[0][0]) and np.array_equal(K, out[0][1])
 
 
-@pytest.mark.parametrize("case", ["one_hex8_nh", "one_tet10_poisson", "all_dirichlet", "one_free_node"])
+@pytest.mark.parametrize("case", ["one_hex8_nh", "one_tet10_poisson", "all_dirichlet", "one_free_node",
+                                  "fan_tet10_nh"])
 def test_edge_meshes(fec, orc, gpu, case):
-    """Smallest meshes and degenerate DOF sets: one element, every dof constrained (empty system), one unknown."""
+    """Smallest meshes and degenerate DOF sets: one element, every dof constrained (empty system), one unknown; and a
+    vertex row wider than one reduction sweep."""
     print(f"[gpu edge] {case}", flush=True)
     W = fec.workloads
     if case == "one_hex8_nh":
@@ -180,6 +182,9 @@ def test_edge_meshes(fec, orc, gpu, case):
         X = np.vstack([v] + [0.5 * (v[a] + v[b]) for a, b in edges])
         mesh = fec.TetMesh(X, np.arange(1, 11, dtype=np.int64)[None, :])
         wl = W.Workload(case, mesh, fec.Poisson("sines", 1.0), 1, np.cos(np.arange(10.0)))
+    elif case == "fan_tet10_nh":
+        mesh = fec.fan_tet10_mesh(32)
+        wl = W.Workload(case, mesh, fec.NeoHookean(1.0, 0.5), 3, 0.02 * np.sin(np.arange(3.0 * mesh.num_nodes)))
     else:
         mesh = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (3, 3, 3))
         dd = np.arange(1, 28, dtype=np.int64) if case == "all_dirichlet" else np.delete(np.arange(1, 28), 13)
@@ -202,6 +207,8 @@ def test_edge_meshes(fec, orc, gpu, case):
     KV = fec.hvp(asm, Vu).cpu().numpy()
     assert len(R) == len(unknown) and K.values.numel() == len(vals)
     assert np.array_equal(K.rowptr.cpu().numpy() + 1, rp) and np.array_equal(K.colval.cpu().numpy() + 1, cv)
+    if case == "fan_tet10_nh":
+        assert np.diff(rp).max() // 3 == 165
     if len(unknown):
         assert rel(R, R_ref) < TOL["R"] and rel(K.values.cpu().numpy(), vals) < TOL["K"]
         assert rel(KV, KVm_ref) < TOL["KV"]

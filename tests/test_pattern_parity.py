@@ -25,11 +25,13 @@ def _cases(fec):
         "hex8_nf1_dirichlet_scattered": (hexm, 1, np.array([1, 7, 8, 33, 50, 51, 79], np.int64)),
         "tet10_nf3": (tet.__class__(tet.coords, tet.conn), 3, np.zeros(0, np.int64)),
         "tet10_nf1_dirichlet": (tet, 1, fec.boundary_nodes(tet)),
+        "tet10_fan_nf3": (fec.fan_tet10_mesh(32), 3, np.zeros(0, np.int64)),  # a 165-block row
     }
 
 
 @pytest.mark.parametrize("name", ["quad4_nf1_dirichlet", "hex8_nf1", "hex8_nf3", "hex8_nf3_dirichlet",
-                                  "hex8_nf1_dirichlet_scattered", "tet10_nf3", "tet10_nf1_dirichlet"])
+                                  "hex8_nf1_dirichlet_scattered", "tet10_nf3", "tet10_nf1_dirichlet",
+                                  "tet10_fan_nf3"])
 def test_pattern_and_map_bit_exact(fec, orc, name):
     mesh, nf, dd = _cases(fec)[name]
     hp = fec.HostPattern(mesh, nf, dd)
