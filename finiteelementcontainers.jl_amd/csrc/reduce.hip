@@ -229,6 +229,73 @@ void reduce_rows_kernel(ReduceArgs a) {
   }
 }
 
+// Scalar hex8 light rows (the Poisson matrix): the gather leaves word b of every staged incidence in lane b's
+// registers, so each lane adds its words straight into FP64 accumulators of the row's column slots in LDS, one
+// incidence after the other in ascending order. Within one incidence the lanes hit distinct slots (a node appears once
+// per element), so the plain read-add-write steps never collide, and each slot sums its blocks in the order of the
+// slot-owner kernel above, with the same bits, but without that kernel's scan of every incidence's slot bytes for
+// every slot (814 VALU instructions per wavefront there, most of its time on the Poisson 128^3 matrix).
+template <int SMAX>
+__global__ __launch_bounds__(RED_THREADS) void reduce_scalar_hex8_kernel(ReduceArgs a) {
+  constexpr int NNPE = 8, LPR = 8, NPW = RED_THREADS / LPR, EW = NNPE * (NNPE + 1) / 2;
+  static_assert(CH == LPR, "one staged incidence per lane");
+  __shared__ int16_t s_off[NNPE][NNPE];
+  __shared__ int32_t s_v[NPW][CH];
+  __shared__ uint32_t s_w[NPW][CH][2];
+  __shared__ float s_r[NPW][CH];
+  __shared__ double s_acc[NPW][SMAX + 1];  // + 1: the rows of one wavefront start on different banks
+  for (int i = threadIdx.x; i < NNPE * NNPE; i += RED_THREADS) (&s_off[0][0])[i] = a.kel_off[i];
+  const int64_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
+  const int64_t pos = a.p0 + chunk * NPW + grp;
+  const bool valid = pos < a.p1;
+  const int4 ri = valid ? a.rowinfo[pos] : make_int4(0, 0, 0, 0);
+  const int64_t n = ri.x;
+  const int32_t c0 = ri.y, ninc = ri.w & 0xFFFF;
+  const int deg = ri.w >> 16;  // 0 for a node with Dirichlet dofs
+  const int cnt = min(CH, ninc);
+#pragma unroll
+  for (int k = lane; k < SMAX; k += LPR) s_acc[grp][k] = 0.0;
+  // stage the incidences (padded rows: the 8 slots at 8 pos, loaded without waiting for the descriptor)
+  const bool stage = a.padded ? (valid && lane < CH) : (lane < cnt);
+  const int64_t c = a.padded ? pos * CH + lane : (int64_t)c0 + lane;
+  if (stage) {
+    const int32_t v = a.vent[c];
+    s_v[grp][lane] = v;
+    const uint32_t* sl = reinterpret_cast<const uint32_t*>(a.vslot) + c * 2;
+    s_w[grp][lane][0] = sl[0];
+    s_w[grp][lane][1] = sl[1];
+    if (a.R) s_r[grp][lane] = a.Rel[v];
+  }
+  __syncthreads();
+  if (deg > 0 && cnt > 0) {
+    float w[CH];  // word `lane` (block K_a,lane) of every staged incidence, all loads in flight
+#pragma unroll
+    for (int t = 0; t < CH; ++t) {
+      const uint32_t v = (uint32_t)s_v[grp][t < cnt ? t : 0];
+      w[t] = a.Kel[(v / NNPE) * (uint32_t)EW + (uint32_t)s_off[v % NNPE][lane]];
+    }
+#pragma unroll
+    for (int t = 0; t < CH; ++t) {
+      if (t < cnt) {
+        const uint32_t sb = (s_w[grp][t][lane >> 2] >> (8 * (lane & 3))) & 0xFFu;  // this column's slot in the row
+        if (sb != 0xFFu) s_acc[grp][sb] += (double)w[t];
+      }
+    }
+  }
+  __syncthreads();
+  if (a.R && lane == 0) {
+    double racc = 0.0;
+    for (int t = 0; t < cnt; ++t) racc += (double)s_r[grp][t];
+    if (valid) a.R[n] = racc;
+  }
+  for (int k = lane; k < deg; k += LPR) a.vals[(int64_t)ri.z + k] = s_acc[grp][k];
+}
+
+#ifndef RED_SCATTER
+#define RED_SCATTER 1
+#endif
+
 // One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 8 for
 // scalar rows of up to 32 blocks (hex8: 27, in four sweeps; 32 rows per work-group, every lane busy in the gather of
 // the 8-word scalar blocks: 0.46 ms vs 0.575 ms with 16 lanes on Poisson 128^3), 32 for vector rows of up to 32
@@ -280,7 +347,19 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
   // direct slot addressing measured -1 % (scalar rows) and +0.8 % (vector rows) on the matrix reduction
   lo.padded = (NF == 1 && a.npad >= lo.p1) ? 1 : 0;
   hi.padded = 0;
-  int rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
+  int rc = 0;
+  if constexpr (NF == 1 && NNPE == 8) {
+    if (RED_SCATTER && lo.p1 > lo.p0) {
+      if (a.maxdeg_light > 32) return -1;
+      int64_t grid = (lo.p1 - lo.p0 + RED_THREADS / 8 - 1) / (RED_THREADS / 8);
+      grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
+      reduce_scalar_hex8_kernel<32><<<(unsigned)grid, RED_THREADS, 0, s>>>(lo);
+    } else {
+      rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
+    }
+  } else {
+    rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
+  }
   if (rc == 0) rc = launch_lpr<NF, NNPE>(hi, a.maxdeg, s);
   return rc;
 }
@@ -335,10 +414,71 @@ __global__ __launch_bounds__(256) void reduce_vector_kernel(ReduceArgs a) {
   for (int i = 0; i < NF; ++i) a.R[n * NF + i] = acc[i];
 }
 
+// Vector-only reduction with G lanes per row, for rows with many incidences (tet10 vertices: 24): lane l loads the
+// incidences l, l + G, ... of each pass of CHV, all before any add, stages their element rows in LDS, and lane i < NF
+// of the row then adds component i of incidences 0, 1, ... in ascending order: the order, and so the bits, of the
+// thread-per-row kernel above and of the matrix kernel's residual sum.
+template <int NF, int G, int CHV>
+__global__ __launch_bounds__(256) void reduce_vector_group_kernel(ReduceArgs a) {
+  constexpr int RPW = 256 / G, PER = CHV / G;
+  static_assert(CHV % G == 0, "whole incidences per lane");
+  __shared__ float s_r[RPW][CHV][NF];
+  __shared__ int s_maxpass;
+  const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+  const int64_t pos = a.p0 + xcd_chunk(blockIdx.x, gridDim.x) * RPW + grp;
+  const bool valid = pos < a.p1;
+  const int4 ri = valid ? a.rowinfo[pos] : make_int4(0, 0, 0, 0);
+  const int32_t c0 = ri.y, ninc = ri.w & 0xFFFF;
+  if (threadIdx.x == 0) s_maxpass = 0;
+  __syncthreads();
+  if (lane == 0) atomicMax(&s_maxpass, (ninc + CHV - 1) / CHV);
+  __syncthreads();
+  const int maxpass = s_maxpass;
+  double acc = 0.0;
+  for (int pass = 0; pass < maxpass; ++pass) {
+    const int base = pass * CHV;
+    int32_t v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int t = base + lane + G * k;
+      v[k] = t < ninc ? a.vent[c0 + t] : -1;
+    }
+    float r[PER][NF];
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+#pragma unroll
+      for (int i = 0; i < NF; ++i) r[k][i] = v[k] >= 0 ? a.Rel[(int64_t)v[k] * NF + i] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+#pragma unroll
+      for (int i = 0; i < NF; ++i) s_r[grp][lane + G * k][i] = r[k][i];
+    __syncthreads();
+    if (lane < NF) {
+      const int cnt = min(CHV, ninc - base);
+      for (int t = 0; t < cnt; ++t) acc += (double)s_r[grp][t][lane];
+    }
+    if (pass + 1 < maxpass) __syncthreads();
+  }
+  if (valid && lane < NF) a.R[(int64_t)ri.x * NF + lane] = acc;
+}
+
+#ifndef RED_VEC_GROUP
+#define RED_VEC_GROUP 1
+#endif
+
 int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s) {
   if (a.p1 <= a.p0) return 0;
   if (!a.vals) {
     if (!a.R) return 0;
+    if (RED_VEC_GROUP && a.nnpe == 10) {
+      constexpr int G = 8;
+      int64_t grid = (a.p1 - a.p0 + 256 / G - 1) / (256 / G);
+      grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
+      if (nf == 1) reduce_vector_group_kernel<1, G, 24><<<(unsigned)grid, 256, 0, s>>>(a);
+      else if (nf == 3) reduce_vector_group_kernel<3, G, 24><<<(unsigned)grid, 256, 0, s>>>(a);
+      else return -1;
+      return 0;
+    }
     int64_t grid = (a.p1 - a.p0 + 255) / 256;
     grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
     if (nf == 1) reduce_vector_kernel<1><<<(unsigned)grid, 256, 0, s>>>(a);
