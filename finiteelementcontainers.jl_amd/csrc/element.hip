@@ -45,14 +45,38 @@ struct ElemLDS {
       float C[GS ? Mat<PHYS>::NC : 1];                          // compact form: the whole tangent data
     };
   };
-  // the element matrix blocks are staged over the quadrature data once that has been consumed
-  union {
-    QP qp[T::NQ];
-    float K[(DO_K || DO_ASM) ? T::NPAIRS * NF * NF : 1];
-  };
+  static constexpr int KW = (DO_K || DO_ASM) ? T::NPAIRS * NF * NF : 1;  // staged element-matrix words
 };
 
 constexpr int ELEM_THREADS = 256;  // four wavefronts, 256 / G elements per work-group
+
+// Quadrature data of the work-group's elements, and over it, once consumed, their element-matrix blocks: those of one
+// wavefront's elements are contiguous, in element storage order, so the wavefront copies them to HBM as 16-byte
+// vectors (one 1 KB store per instruction) with no per-word element/offset arithmetic.
+// One element's quadrature points, padded to a stride of 8 (mod 16) words: the 8 elements of a hex8 wavefront (4 of a
+// tet10 one) then start on distinct LDS banks when their node lanes read the same point (J2 hex8, 352 words unpadded,
+// put all 8 on two bank offsets: its element kernel ran 7 % slower than with the per-element layout).
+template <class QP, int NQ, int PADW>
+struct QPBlock {
+  QP q[NQ];
+  float pad[PADW];
+};
+template <class QP, int NQ>
+struct QPBlock<QP, NQ, 0> {
+  QP q[NQ];
+};
+template <class QP, int NQ>
+constexpr int qp_pad() {
+  constexpr int w = (int)(NQ * sizeof(QP) / sizeof(float));
+  return ((8 - w % 16) % 16 + 16) % 16;
+}
+template <class LDS, int EPW, int NQ>
+struct ElemWork {
+  union {
+    QPBlock<typename LDS::QP, NQ, qp_pad<typename LDS::QP, NQ>()> qp[EPW];
+    float K[EPW * LDS::KW];
+  };
+};
 
 template <int KIND, int NF, int PHYS, int MODE>
 __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
@@ -65,6 +89,8 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   constexpr bool DO_G = (MODE & MODE_QPGRAD) != 0;
   constexpr bool NEED_C = LDS::NEED_C, GS = LDS::GS;
   __shared__ LDS lds[EPW];
+  __shared__ __attribute__((aligned(16))) ElemWork<LDS, EPW, NQ> work;
+  constexpr int KW = LDS::KW;
 
   const int lane = threadIdx.x % G, grp = threadIdx.x / G;
   const int64_t e = args.ebeg + (int64_t)blockIdx.x * EPW + grp;  // element storage position
@@ -164,7 +190,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
       Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
     }
     if (!(det > 0.0f)) bad = true;
-    auto& Q = L.qp[q];
+    auto& Q = work.qp[grp].q[q];
     float g[NNPE][ND];
 #pragma unroll
     for (int a = 0; a < NNPE; ++a)
@@ -282,7 +308,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
       float out[NF] = {};
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        const auto& Q = L.qp[q];
+        const auto& Q = work.qp[grp].q[q];
         float Na = 0.0f, dNa[ND];
         if constexpr (DO_R && PHYS == FECX_POISSON) {
           float xi[ND];
@@ -310,7 +336,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         for (int c = 0; c < NF * NF; ++c) acc[p][c] = 0.0f;
 #pragma unroll 2
       for (int q = 0; q < NQ; ++q) {
-        const auto& Q = L.qp[q];
+        const auto& Q = work.qp[grp].q[q];
         float C[M::NC];
         float va[M::NV], sa[M::NA];
         load_c(Q, C);
@@ -355,7 +381,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
       float acc[NF * NF] = {};
 #pragma unroll 2
       for (int q = 0; q < NQ; ++q) {
-        const auto& Q = L.qp[q];
+        const auto& Q = work.qp[grp].q[q];
         float C[M::NC];
         float va[M::NV], sa[M::NA];
         load_c(Q, C);
@@ -373,7 +399,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     __syncthreads();
     if (valid && lane < NNPE) {
       const int cnt = lane_cnt(NNPE, lane);
-      float* dst = L.K + pair_lane_off(NNPE, lane) * (NF * NF);
+      float* dst = work.K + grp * KW + pair_lane_off(NNPE, lane) * (NF * NF);
 #pragma unroll
       for (int p = 0; p < MAXP; ++p)
         if (p < cnt)
@@ -385,7 +411,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
       float kv[NF] = {};
       for (int b = 0; b < NNPE; ++b) {
         const PairRef pr = pair_ref(NNPE, lane, b);
-        const float* blk = L.K + pr.slot * (NF * NF);
+        const float* blk = work.K + grp * KW + pr.slot * (NF * NF);
 #pragma unroll
         for (int i = 0; i < NF; ++i)
 #pragma unroll
@@ -401,7 +427,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     __syncthreads();
     if (valid && lane < NNPE) {
       const int cnt = lane_cnt(NNPE, lane);
-      float* dst = L.K + pair_lane_off(NNPE, lane) * (NF * NF);
+      float* dst = work.K + grp * KW + pair_lane_off(NNPE, lane) * (NF * NF);
 #pragma unroll
       for (int p = 0; p < MAXP; ++p)
         if (p < cnt)
@@ -414,10 +440,18 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     const int wl = threadIdx.x % 64, wv = threadIdx.x / 64;
     const int64_t e0 = args.ebeg + (int64_t)blockIdx.x * EPW + wv * EPWAVE;
     const int64_t nvalid = args.ne - e0 < EPWAVE ? args.ne - e0 : EPWAVE;
+    static_assert(EW == KW && (EPWAVE * EW) % 4 == 0, "a wavefront's blocks start 16-byte aligned in LDS");
     if (nvalid > 0) {
       float* out = args.Kel + e0 * EW;
+      const float* src = work.K + wv * EPWAVE * EW;
       const int total = (int)nvalid * EW;
-      for (int idx = wl; idx < total; idx += 64) out[idx] = lds[wv * EPWAVE + idx / EW].K[idx % EW];
+      int done = 0;
+      if (((e0 * EW) & 3) == 0) {  // 16-byte aligned in HBM too (always, unless a pipelined piece starts unaligned)
+        done = total & ~3;
+        for (int i = 4 * wl; i < done; i += 256)
+          *reinterpret_cast<float4*>(out + i) = *reinterpret_cast<const float4*>(src + i);
+      }
+      for (int idx = done + wl; idx < total; idx += 64) out[idx] = src[idx];
     }
   }
   if (bad) atomicOr(args.flag, 1);
