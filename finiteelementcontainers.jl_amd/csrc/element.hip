@@ -49,28 +49,10 @@ struct ElemLDS {
 };
 
 constexpr int ELEM_THREADS = 256;  // four wavefronts, 256 / G elements per work-group
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Quadrature data of the work-group's elements, and over it, once consumed, their element-matrix blocks: those of one
 // wavefront's elements are contiguous, in element storage order, so the wavefront copies them to HBM as 16-byte
 // vectors (one 1 KB store per instruction) with no per-word element/offset arithmetic.
-// One element's quadrature points, padded to a stride of 8 (mod 16) words: the 8 elements of a hex8 wavefront (4 of a
-// tet10 one) then start on distinct LDS banks when their node lanes read the same point (J2 hex8, 352 words unpadded,
-// put all 8 on two bank offsets: its element kernel ran 7 % slower than with the per-element layout).
-template <class QP, int NQ, int PADW>
-struct QPBlock {
-  QP q[NQ];
-  float pad[PADW];
-};
-template <class QP, int NQ>
-struct QPBlock<QP, NQ, 0> {
-  QP q[NQ];
-};
-template <class QP, int NQ>
-constexpr int qp_pad() {
-  constexpr int w = (int)(NQ * sizeof(QP) / sizeof(float));
-  return ((8 - w % 16) % 16 + 16) % 16;
-}
 template <class LDS, int EPW, int NQ>
 struct ElemWork {
   union {

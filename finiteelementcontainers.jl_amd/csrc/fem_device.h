@@ -385,4 +385,24 @@ struct Mat<FECX_NH_SURROGATE> : Mat<FECX_NEOHOOKEAN> {
   static constexpr bool FROM_QP = true;
 };
 
+// ------------------------------------------------------------------ LDS staging shared by the element and mass kernels
+typedef float f32x4 __attribute__((ext_vector_type(4)));  // 16-byte moves (nontemporal stores need a vector type)
+
+// One element's quadrature points, padded to a stride of 8 (mod 16) words: the 8 elements of a hex8 wavefront (4 of a
+// tet10 one) then start on distinct LDS banks when their node lanes read the same point (J2 hex8, 352 words unpadded,
+// put all 8 on two bank offsets: its element kernel ran 7 % slower than with the per-element layout).
+template <class QP, int NQ, int PADW>
+struct QPBlock {
+  QP q[NQ];
+  float pad[PADW];
+};
+template <class QP, int NQ>
+struct QPBlock<QP, NQ, 0> {
+  QP q[NQ];
+};
+template <class QP, int NQ>
+constexpr int qp_pad() {
+  constexpr int w = (int)(NQ * sizeof(QP) / sizeof(float));
+  return ((8 - w % 16) % 16 + 16) % 16;
+}
 }  // namespace fecx

@@ -22,13 +22,20 @@ struct MassLDS {
     float w;
     float N[T::NNPE];
   };
-  union {
-    QP qp[T::NQ];
-    float K[WHAT == MASS_MATRIX ? T::NPAIRS * NF * NF : 1];
-  };
+  static constexpr int KW = WHAT == MASS_MATRIX ? T::NPAIRS * NF * NF : 1;
 };
 
 constexpr int MASS_THREADS = 256;
+
+// The work-group's quadrature data and, over it once consumed, its element-matrix blocks, contiguous per wavefront
+// (the element kernels' layout: 16-byte copies to HBM, per-element qp blocks padded to 8 (mod 16) words).
+template <class LDS, int EPW, int NQ>
+struct MassWork {
+  union {
+    QPBlock<typename LDS::QP, NQ, qp_pad<typename LDS::QP, NQ>()> qp[EPW];
+    float K[EPW * LDS::KW];
+  };
+};
 
 template <int KIND, int NF, int WHAT>
 __global__ __launch_bounds__(256) void mass_kernel(ElementArgs args, float rho) {
@@ -37,6 +44,8 @@ __global__ __launch_bounds__(256) void mass_kernel(ElementArgs args, float rho) 
   constexpr int ND = T::ND, NNPE = T::NNPE, NQ = T::NQ, G = T::G, EPW = MASS_THREADS / G, NDOF = NNPE * NF;
   constexpr int MAXP = T::MAXP;
   __shared__ LDS lds[EPW];
+  __shared__ __attribute__((aligned(16))) MassWork<LDS, EPW, NQ> work;
+  constexpr int KW = LDS::KW;
 
   const int lane = threadIdx.x % G, grp = threadIdx.x / G;
   const int64_t e = args.ebeg + (int64_t)blockIdx.x * EPW + grp;  // element storage position
@@ -93,9 +102,9 @@ __global__ __launch_bounds__(256) void mass_kernel(ElementArgs args, float rho) 
             J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
     }
     if (!(det > 0.0f)) bad = true;
-    L.qp[q].w = rho * det * qweight<KIND>();
+    work.qp[grp].q[q].w = rho * det * qweight<KIND>();
 #pragma unroll
-    for (int a = 0; a < NNPE; ++a) L.qp[q].N[a] = N[a];
+    for (int a = 0; a < NNPE; ++a) work.qp[grp].q[q].N[a] = N[a];
   }
   __syncthreads();
 
@@ -107,8 +116,8 @@ __global__ __launch_bounds__(256) void mass_kernel(ElementArgs args, float rho) 
       float s = 0.0f;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        const float na = L.qp[q].N[a];
-        s = fmaf(L.qp[q].w, WHAT == MASS_DIAG ? na * na : na, s);
+        const float na = work.qp[grp].q[q].N[a];
+        s = fmaf(work.qp[grp].q[q].w, WHAT == MASS_DIAG ? na * na : na, s);
       }
 #pragma unroll
       for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + a * NF + i] = s;
@@ -117,7 +126,7 @@ __global__ __launch_bounds__(256) void mass_kernel(ElementArgs args, float rho) 
       for (int b = 0; b < NNPE; ++b) {
         float m = 0.0f;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) m = fmaf(L.qp[q].w * L.qp[q].N[a], L.qp[q].N[b], m);
+        for (int q = 0; q < NQ; ++q) m = fmaf(work.qp[grp].q[q].w * work.qp[grp].q[q].N[a], work.qp[grp].q[q].N[b], m);
 #pragma unroll
         for (int i = 0; i < NF; ++i) out[i] = fmaf(m, L.v[b][i], out[i]);
       }
@@ -131,7 +140,7 @@ __global__ __launch_bounds__(256) void mass_kernel(ElementArgs args, float rho) 
         if (p < cnt) {
           const int b = a + p < NNPE ? a + p : a + p - NNPE;
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) m = fmaf(L.qp[q].w * L.qp[q].N[a], L.qp[q].N[b], m);
+          for (int q = 0; q < NQ; ++q) m = fmaf(work.qp[grp].q[q].w * work.qp[grp].q[q].N[a], work.qp[grp].q[q].N[b], m);
         }
         kreg[p] = m;
       }
@@ -143,7 +152,7 @@ __global__ __launch_bounds__(256) void mass_kernel(ElementArgs args, float rho) 
     __syncthreads();
     if (valid && lane < NNPE) {
       const int cnt = lane_cnt(NNPE, lane);
-      float* dst = L.K + pair_lane_off(NNPE, lane) * (NF * NF);
+      float* dst = work.K + grp * KW + pair_lane_off(NNPE, lane) * (NF * NF);
 #pragma unroll
       for (int p = 0; p < MAXP; ++p)
         if (p < cnt)
@@ -156,10 +165,18 @@ __global__ __launch_bounds__(256) void mass_kernel(ElementArgs args, float rho) 
     const int wl = threadIdx.x % 64, wv = threadIdx.x / 64;
     const int64_t e0 = args.ebeg + (int64_t)blockIdx.x * EPW + wv * EPWAVE;
     const int64_t nvalid = args.ne - e0 < EPWAVE ? args.ne - e0 : EPWAVE;
+    static_assert(EW == KW && (EPWAVE * EW) % 4 == 0, "a wavefront's blocks start 16-byte aligned in LDS");
     if (nvalid > 0) {
       float* out = args.Kel + e0 * EW;
+      const float* src = work.K + wv * EPWAVE * EW;
       const int total = (int)nvalid * EW;
-      for (int idx = wl; idx < total; idx += 64) out[idx] = lds[wv * EPWAVE + idx / EW].K[idx % EW];
+      int done = 0;
+      if (((e0 * EW) & 3) == 0) {  // 16-byte aligned in HBM too
+        done = total & ~3;
+        for (int i = 4 * wl; i < done; i += 256)
+          __builtin_nontemporal_store(*reinterpret_cast<const f32x4*>(src + i), reinterpret_cast<f32x4*>(out + i));
+      }
+      for (int idx = done + wl; idx < total; idx += 64) out[idx] = src[idx];
     }
   }
   if (bad) atomicOr(args.flag, 1);
