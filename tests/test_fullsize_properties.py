@@ -31,7 +31,7 @@ def _spmv(fec, asm, vals, x):
     return y
 
 
-@pytest.mark.parametrize("name,n", [("poisson_hex8", 128), ("neohookean_hex8", 96)])
+@pytest.mark.parametrize("name,n", [("poisson_hex8", 128), ("neohookean_hex8", 96), ("neohookean_tet10", 80)])
 def test_full_size_properties(fec, gpu, name, n):
     print(f"[fullsize] {name} {n}", flush=True)
     wl = fec.workloads.CONFIGS[name](n)
