@@ -102,6 +102,24 @@ int pipeline_chunks(fecx_ctx* ctx, int mode) {
   return ctx->chunks > 0 ? ctx->chunks : 1;
 }
 
+// The numeric flag before a launch: cleared on the device only if the last read-back did not find it clear.
+int* arm_flag(fecx_ctx* ctx) {
+  int* flag = get<int>(ctx, "flag");
+  if (ctx->flag_dirty) zero_int_kernel<<<1, 1, 0, ctx->stream>>>(flag);
+  ctx->flag_dirty = true;  // until this call reads it back as clear
+  if (!ctx->flag_host) HIPCHK(hipHostMalloc((void**)&ctx->flag_host, sizeof(int), hipHostMallocDefault));
+  return flag;
+}
+
+// Reads the flag back (pinned word, stream-ordered after the kernels) and waits for the stream.
+int read_flag(fecx_ctx* ctx, const int* flag) {
+  HIPCHK(hipMemcpyAsync(ctx->flag_host, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int h = *ctx->flag_host;
+  ctx->flag_dirty = h != 0;
+  return h;
+}
+
 // Runs the element kernel for `mode`, then the requested reductions; syncs and checks the flag.
 // With more than one chunk, the Morton-ordered row nodes are split into `k` ranges of positions [p_j, p_j+1); element
 // chunk j is the storage range of the elements whose smallest node rank lies in that range (contiguous, elements are
@@ -113,8 +131,7 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   check_ready(ctx);
   require(U != nullptr, FECX_ERR_ARG, "U is NULL");
   Prof prof(ctx);
-  int* flag = get<int>(ctx, "flag");
-  zero_int_kernel<<<1, 1, 0, ctx->stream>>>(flag);
+  int* flag = arm_flag(ctx);
   fecx::ElementArgs ea = element_args(ctx, U, V);
   const bool surrogate_phys = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
   const bool surrogate = surrogate_phys && ctx->sg_enabled;
@@ -169,9 +186,7 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
     require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
     HIPCHK(hipGetLastError());
   }
-  int hflag = 0;
-  HIPCHK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int hflag = read_flag(ctx, flag);
   prof.finish();
   if (surrogate) fecx::surrogate_qp_finish(ctx);
   require(hflag == 0, FECX_ERR_NUMERIC, "element kernel met a non-positive Jacobian determinant or a NaN");
@@ -185,8 +200,7 @@ void assemble_mass(fecx_ctx* ctx, int what, double density, const double* V, dou
           "mesh, dofs and pattern must be set before a mass assembly");
   require(std::isfinite(density), FECX_ERR_ARG, "density must be finite");
   Prof prof(ctx);
-  int* flag = get<int>(ctx, "flag");
-  zero_int_kernel<<<1, 1, 0, ctx->stream>>>(flag);
+  int* flag = arm_flag(ctx);
   fecx::ElementArgs ea = element_args(ctx, nullptr, V);
   int rc = 0;
   prof.run("mass_element", ctx->stream, [&] { rc = fecx::launch_mass(ctx->kind, ctx->nf, what, ea, (float)density, ctx->stream); });
@@ -198,9 +212,7 @@ void assemble_mass(fecx_ctx* ctx, int what, double density, const double* V, dou
   prof.run("reduce", ctx->stream, [&] { rc = fecx::launch_reduce(ctx->nf, ra, ctx->stream); });
   require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
   HIPCHK(hipGetLastError());
-  int hflag = 0;
-  HIPCHK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int hflag = read_flag(ctx, flag);
   prof.finish();
   require(hflag == 0, FECX_ERR_NUMERIC, "mass kernel met a non-positive Jacobian determinant");
 }
@@ -232,6 +244,7 @@ int fecx_destroy(fecx_ctx* ctx) {
     if (kv.second.p) (void)hipFree(kv.second.p);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+  if (ctx->flag_host) (void)hipHostFree(ctx->flag_host);
   delete ctx;
   return FECX_OK;
 }

@@ -201,6 +201,10 @@ struct fecx_ctx {
   int chunks = 0;  // 0: chosen from the element storage size
   hipStream_t stream2 = nullptr;
   std::vector<hipEvent_t> events;
+  // numeric error flag: device word set by the kernels, read back through a pinned host word after every assembly;
+  // cleared on the device only when the last read-back did not find it clear (an error, or a call that stopped early)
+  int* flag_host = nullptr;
+  bool flag_dirty = true;
   // profiling
   bool profiling = false;
   std::vector<std::string> prof_names;

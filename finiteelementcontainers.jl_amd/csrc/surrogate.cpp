@@ -414,6 +414,7 @@ int fecx_qp_gradients(fecx_ctx* ctx, const double* U, float* Hq) {
     ea.mp = ctx->mp;
     ea.hq = (float*)dev_atleast(ctx, "hq", sizeof(float) * nqp * 9);
     HIPCHK(hipMemsetAsync(ea.flag, 0, sizeof(int), ctx->stream));
+    ctx->flag_dirty = true;  // not read back here: the next assembly clears it first
     require(fecx::launch_element(ctx->kind, ctx->nf, fecx::MODE_QPGRAD, ea, ctx->stream) == 0, FECX_ERR_UNSUPPORTED,
             "no gradient kernel for this element kind");
     // storage order -> element order: row e of Hq is row epos[e] of the storage-ordered buffer

@@ -148,6 +148,26 @@ def test_numeric_error_flag(fec, gpu):
     assert ei.value.code == -5
 
 
+def test_numeric_error_flag_recovers(fec, gpu):
+    """The error flag is cleared on the device only after a call that found it set: on one context a bad call
+    (u_x = -2x, so F_xx = -1 and J < 0 at every point) raises, the next good call succeeds, a bad one raises again."""
+    wl = fec.workloads.neohookean_hex8(3)
+    asm, p = _setup(fec, wl, gpu)
+    N = 3 * wl.mesh.num_nodes
+    bad = np.zeros(N)
+    bad[0::3] = -2.0 * wl.mesh.coords[:, 0]
+    Ub = torch.from_numpy(bad).to(gpu)
+    Ug = torch.zeros(N, dtype=torch.float64, device=gpu)
+    for U, ok in [(Ub, False), (Ug, True), (Ug, True), (Ub, False), (Ub, False), (Ug, True)]:
+        if ok:
+            fec.assemble_vector_and_stiffness_(asm, U, p)
+            assert float(torch.linalg.norm(fec.residual(asm))) == 0.0  # undeformed: no residual
+        else:
+            with pytest.raises(fec.FECXError) as ei:
+                fec.assemble_vector_and_stiffness_(asm, U, p)
+            assert ei.value.code == -5
+
+
 @pytest.mark.parametrize("name", ["neohookean_hex8", "poisson_tet10", "poisson_hex8"])
 def test_pipelined_chunks_bit_identical(fec, gpu, name):
     """Pipelined matrix assembly (element and reduction pieces on two streams) gives the same bits as one piece."""
