@@ -8,6 +8,11 @@
 // adjacent lanes writing adjacent columns.
 // Vector: the same groups sum the node's incident element rows into the residual field (the atomic scatter of
 // src/assemblers/Assemblers.jl:72-87 restated as an owner-computes gather).
+// Three kernels share this contract:
+//   reduce_rows_kernel          slot-owner lanes (vector rows, scalar tet10/quad4 rows, every heavy row);
+//   reduce_scalar_hex8_kernel   scalar hex8 light rows: lanes add their words into LDS slot accumulators instead;
+//   reduce_vector_kernel / reduce_vector_group_kernel   calls without CSR values (one thread, or 8 lanes, per row).
+// All sum a slot's (or a row's) contributions in ascending incidence order, so every form gives the same bits.
 #include <algorithm>
 
 #include "fecx_internal.h"
@@ -23,7 +28,7 @@ constexpr int CH = 8;            // incidences staged per pass
 #define RED_NF3_LPR 32           // lanes per vector row of at most 32 blocks
 #endif
 #ifndef RED_HEAVY_LPR
-#define RED_HEAVY_LPR 128        // lanes per row of 65..128 blocks (128: one slot per lane; 64: two sweeps)
+#define RED_HEAVY_LPR 128        // lanes per row of 65..256 blocks (one slot per lane and sweep, two sweeps past 128)
 #endif
 #ifndef RED_LB
 #define RED_LB 4                 // vector rows: incidences whose block loads are in flight together
