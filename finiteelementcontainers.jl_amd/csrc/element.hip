@@ -140,21 +140,45 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   }
   __syncthreads();
 
-  // ---------------- quadrature points (lane = q)
-  if (valid && lane < NQ) {
-    const int q = lane;
+  // ---------------- quadrature points: S lanes per point, lane S q + r handling the nodes r, r + S, r + 2S, ...
+  // (tet10: 4 points on 16 lanes, S = 4; quad4/hex8: S = 1). The node sums (Jacobian, field gradient, source point,
+  // action gradient) are completed across the point's 4 lanes by two DPP quad permutations (xor 1, then xor 2: VALU
+  // operations, no LDS round trip), which leave the same bits on every lane of the point, so each lane evaluates the
+  // material itself and writes the quadrature data of its own nodes.
+  constexpr int S = KIND == FECX_TET10 ? G / NQ : 1;
+  constexpr int NMY = (NNPE + S - 1) / S;
+  static_assert(NQ * S <= G && (S == 1 || S == 4), "the lanes of an element cover its quadrature points in quads");
+  auto lane_sum = [](float& v) {
+    if constexpr (S == 4) {
+      v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
+      v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+    }
+  };
+  if (valid && lane < NQ * S) {
+    const int q = lane / S, r = lane % S;
+    auto mine = [&](int k) { return S == 1 || r + S * k < NNPE; };
     float xi[ND];
     qpoint<KIND>(q, xi);
-    float N[NNPE], dN[NNPE][ND];
+    float N[NMY], dN[NMY][ND];
 #pragma unroll
-    for (int a = 0; a < NNPE; ++a) shape_node<KIND>(a, xi, N[a], dN[a]);
+    for (int k = 0; k < NMY; ++k) {
+      N[k] = 0.0f;
+#pragma unroll
+      for (int j = 0; j < ND; ++j) dN[k][j] = 0.0f;
+      if (mine(k)) shape_node<KIND>(r + S * k, xi, N[k], dN[k]);
+    }
     float J[ND][ND] = {};
 #pragma unroll
-    for (int a = 0; a < NNPE; ++a)
+    for (int k = 0; k < NMY; ++k)
+      if (mine(k))
 #pragma unroll
-      for (int i = 0; i < ND; ++i)
+        for (int i = 0; i < ND; ++i)
 #pragma unroll
-        for (int j = 0; j < ND; ++j) J[i][j] += L.x[a][i] * dN[a][j];
+          for (int j = 0; j < ND; ++j) J[i][j] += L.x[r + S * k][i] * dN[k][j];
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) lane_sum(J[i][j]);
     float Ji[ND][ND], det;
     if constexpr (ND == 2) {
       det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
@@ -174,29 +198,34 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     }
     if (!(det > 0.0f)) bad = true;
     auto& Q = work.qp[grp].q[q];
-    float g[NNPE][ND];
+    float g[NMY][ND];
 #pragma unroll
-    for (int a = 0; a < NNPE; ++a)
+    for (int k = 0; k < NMY; ++k)
 #pragma unroll
       for (int j = 0; j < ND; ++j) {
-        float s = 0;
+        float t = 0;
 #pragma unroll
-        for (int k = 0; k < ND; ++k) s += dN[a][k] * Ji[k][j];
-        g[a][j] = s;
+        for (int i = 0; i < ND; ++i) t += dN[k][i] * Ji[i][j];
+        g[k][j] = t;
       }
     const float w = det * qweight<KIND>();
-    Q.w = w;
+    if (r == 0) Q.w = w;
     float H[D] = {};
 #pragma unroll
-    for (int a = 0; a < NNPE; ++a)
+    for (int k = 0; k < NMY; ++k)
+      if (mine(k))
 #pragma unroll
-      for (int i = 0; i < NF; ++i)
+        for (int i = 0; i < NF; ++i)
 #pragma unroll
-        for (int j = 0; j < ND; ++j) H[i * ND + j] += L.u[a][i] * g[a][j];
+          for (int j = 0; j < ND; ++j) H[i * ND + j] += L.u[r + S * k][i] * g[k][j];
+#pragma unroll
+    for (int c = 0; c < D; ++c) lane_sum(H[c]);
     if constexpr (DO_G) {  // quadrature-point gradients only, element storage order
-      float* o = args.hq + ((int64_t)e * NQ + q) * D;
+      if (r == 0) {
+        float* o = args.hq + ((int64_t)e * NQ + q) * D;
 #pragma unroll
-      for (int c = 0; c < D; ++c) o[c] = H[c];
+        for (int c = 0; c < D; ++c) o[c] = H[c];
+      }
     } else {
     float P[D], C[M::NC];
     if constexpr (M::FROM_QP) {  // stress and tangent data from the batched (surrogate) material pass
@@ -210,14 +239,14 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
       if constexpr (PHYS == FECX_J2) {
         const int64_t sidx = ((int64_t)args.eperm[e] * NQ + q) * 7;
 #pragma unroll
-        for (int s = 0; s < 7; ++s) so[s] = args.st_old[sidx + s];
+        for (int st = 0; st < 7; ++st) so[st] = args.st_old[sidx + st];
       }
       M::template eval<ND>(args.mp, H, so, sn, P, C, bad);
       if constexpr (PHYS == FECX_J2) {
-        if (DO_R && args.st_new) {
+        if (DO_R && args.st_new && r == 0) {
           const int64_t sidx = ((int64_t)args.eperm[e] * NQ + q) * 7;
 #pragma unroll
-          for (int s = 0; s < 7; ++s) args.st_new[sidx + s] = sn[s];
+          for (int st = 0; st < 7; ++st) args.st_new[sidx + st] = sn[st];
         }
       }
     }
@@ -225,21 +254,25 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     for (int c = 0; c < D; ++c)
       if (!(P[c] == P[c])) bad = true;
     if constexpr (DO_R) {
+      if (r == 0) {
 #pragma unroll
-      for (int c = 0; c < D; ++c) Q.P[c] = P[c];
+        for (int c = 0; c < D; ++c) Q.P[c] = P[c];
+      }
       if constexpr (PHYS == FECX_POISSON) {
         float f = args.mp.p[1];
-        if (args.mp.source_kind == 1) {
+        if (args.mp.source_kind == 1) {  // the same for every lane of the launch
           f *= (float)ND * 9.8696044010893586f;
 #pragma unroll
           for (int d = 0; d < ND; ++d) {
-            float s = 0;
+            float t = 0;
 #pragma unroll
-            for (int a = 0; a < NNPE; ++a) s += N[a] * L.x[a][d];
-            f *= sinpif(L.x0[d] + s);
+            for (int k = 0; k < NMY; ++k)
+              if (mine(k)) t += N[k] * L.x[r + S * k][d];
+            lane_sum(t);
+            f *= sinpif(L.x0[d] + t);
           }
         }
-        Q.src[0] = f;
+        if (r == 0) Q.src[0] = f;
       }
     }
     if constexpr (GS) {
@@ -250,33 +283,42 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
 #pragma unroll
         for (int j = 0; j < ND; ++j) Q.v[a][j] = g[a][j];
     } else if constexpr (DO_K || DO_ASM || DO_D) {
+      if (r == 0) {
 #pragma unroll
-      for (int c = 0; c < M::NS; ++c) Q.S[c] = C[M::S0 + c];
-#pragma unroll
-      for (int a = 0; a < NNPE; ++a) {
-        float va[M::NV];
-        M::template prep<ND>(args.mp, C, g[a], va);
-#pragma unroll
-        for (int c = 0; c < M::NV; ++c) Q.v[a][c] = va[c];
+        for (int c = 0; c < M::NS; ++c) Q.S[c] = C[M::S0 + c];
       }
+#pragma unroll
+      for (int k = 0; k < NMY; ++k)
+        if (mine(k)) {
+          float va[M::NV];
+          M::template prep<ND>(args.mp, C, g[k], va);
+#pragma unroll
+          for (int c = 0; c < M::NV; ++c) Q.v[r + S * k][c] = va[c];
+        }
     } else {
 #pragma unroll
-      for (int a = 0; a < NNPE; ++a)
+      for (int k = 0; k < NMY; ++k)
+        if (mine(k))
 #pragma unroll
-        for (int j = 0; j < ND; ++j) Q.v[a][j] = g[a][j];
+          for (int j = 0; j < ND; ++j) Q.v[r + S * k][j] = g[k][j];
     }
     if constexpr (DO_MF) {
       float Gv[D] = {};
 #pragma unroll
-      for (int a = 0; a < NNPE; ++a)
+      for (int k = 0; k < NMY; ++k)
+        if (mine(k))
 #pragma unroll
-        for (int i = 0; i < NF; ++i)
+          for (int i = 0; i < NF; ++i)
 #pragma unroll
-          for (int j = 0; j < ND; ++j) Gv[i * ND + j] += L.v[a][i] * g[a][j];
+            for (int j = 0; j < ND; ++j) Gv[i * ND + j] += L.v[r + S * k][i] * g[k][j];
+#pragma unroll
+      for (int c = 0; c < D; ++c) lane_sum(Gv[c]);
       float out[D];
       M::template dP<ND>(args.mp, C, Gv, out);
+      if (r == 0) {
 #pragma unroll
-      for (int c = 0; c < D; ++c) Q.P[c] = out[c];
+        for (int c = 0; c < D; ++c) Q.P[c] = out[c];
+      }
     }
     }  // !DO_G
   }
