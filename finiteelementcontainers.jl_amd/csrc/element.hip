@@ -7,6 +7,7 @@
 //                                 differences x_a - x_0 (hi/lo FP32 coordinate split) and u_a - u_0
 //   qp       lane q  (q < nq)   : isoparametric map, shape gradients, field gradient and material point of
 //                                 quadrature point q: stress and a compact form of the tangent, staged in LDS
+//                                 (tet10: lanes 4q .. 4q+3, each with every 4th node)
 //   node     lane a  (a < nnpe) : residual / action rows of node a, and the node-pair blocks K_ab it owns
 //                                 (symmetric storage, one NF x NF block per unordered pair)
 // Each material contracts its tangent with the shape gradients in closed form (K_ab = sum_q w g_a . A . g_b
