@@ -61,11 +61,23 @@ __device__ __forceinline__ int match_slot(const uint32_t* w, int s) {
 //      storage with the transposition of blocks stored as K_ba folded in,
 //   3. sum in FP64, lane s owning column slot s, the staged blocks that land in slot s, in incidence order.
 // The residual sum runs over the same staged rows in the same order.
-// Vector rows ask for 7 waves per SIMD (at most 72 VGPRs): the 32-lane hex8 kernel then holds 70 instead of 77, one
-// occupancy step up, and its reduction measured 1.043 against 1.084 ms on neo-Hookean hex8 96^3. The scalar kernels
-// already fit 8 waves (the tet10 one would spill under the cap).
+// LDS of reduce_rows_kernel<NF, NNPE, LPR, SW, CHK> (the __shared__ arrays below, plus the pass-count word).
+template <int NF, int NNPE, int LPR, int CHK>
+constexpr int reduce_rows_lds() {
+  constexpr int NPW = RED_THREADS / LPR, BW = NNPE * NF * NF, VW = (NNPE + 3) / 4;
+  return NNPE * BW * 2 + NPW * CHK * 4 * (1 + VW + NF + BW) + 4;
+}
+// Vector rows ask for 7 waves per SIMD (at most 72 VGPRs) where LDS leaves room for 7 work-groups per CU (under
+// 22.5 KB each in 2 KB granules): the 32-lane hex8 kernel then holds 70 VGPRs instead of 77, one occupancy step up,
+// and its reduction measured 1.043 against 1.084 ms on neo-Hookean hex8 96^3. The scalar kernels already fit 8 waves
+// (the tet10 one would spill under the cap); the tet10 32-lane kernel is held to 6 by its LDS.
+template <int NF, int NNPE, int LPR, int SW, int CHK>
+constexpr int reduce_rows_waves() {
+  return (NF == 3 && SW == 1 && reduce_rows_lds<NF, NNPE, LPR, CHK>() <= 22528) ? 7 : 1;
+}
 template <int NF, int NNPE, int LPR, int SW, int CHK = CH>
-__global__ __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(NF == 3 && SW == 1 ? 7 : 1, 8)))
+__global__ __launch_bounds__(RED_THREADS)
+__attribute__((amdgpu_waves_per_eu(reduce_rows_waves<NF, NNPE, LPR, SW, CHK>(), 8)))
 void reduce_rows_kernel(ReduceArgs a) {
   static_assert(CHK <= LPR && CHK % 8 == 0, "a row's lanes stage its incidences, 8 per batch");
   constexpr int NPW = RED_THREADS / LPR;  // nodes per work-group
