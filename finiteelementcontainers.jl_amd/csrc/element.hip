@@ -32,11 +32,18 @@ struct ElemLDS {
   // C once; the node lanes form the per-node vectors prep(C, g) themselves. The neo-Hookean and J2 blocks then stage
   // 45 / 44 instead of 60 floats per point: under 53 KB per work-group, three work-groups per CU instead of two.
   static constexpr bool GS = KIND == FECX_HEX8 && NF == 3 && (DO_K || DO_ASM || DO_D) && (Mat<PHYS>::NV > T::ND);
+  // Poisson hex8 residual formed in the quadrature phase: each point's lane computes its contribution to all 8 node
+  // rows and three DPP steps sum them over the element's 8 points, so the flux and source need no LDS (30 KB instead
+  // of 36 KB per work-group: five work-groups per CU instead of four).
+  // The element blocks need only sqrt(w) g_a per point and node (24 floats per point); every Poisson hex8 mode that
+  // forms them stages that (SC), so fused and separate calls give the same bits.
+  static constexpr bool SC = KIND == FECX_HEX8 && PHYS == FECX_POISSON && !DO_MF && (MODE & MODE_QPGRAD) == 0;
+  static constexpr bool QR = SC && DO_R;
   float x[T::NNPE][T::ND];
   float x0[T::ND];
   float u[T::NNPE][NF];
   float v[(DO_MF || DO_ASM) ? T::NNPE : 1][NF];
-  struct QP {
+  struct QPFull {
     float v[T::NNPE][(DO_K || DO_ASM || DO_D) && !GS ? Mat<PHYS>::NV : T::ND];  // v[a][0..ND) = shape gradient g_a
     float w;
     float P[(DO_R || DO_MF) ? D : 1];
@@ -46,6 +53,10 @@ struct ElemLDS {
       float C[GS ? Mat<PHYS>::NC : 1];                          // compact form: the whole tangent data
     };
   };
+  struct QPScaled {
+    float v[T::NNPE][T::ND];  // sqrt(w) g_a
+  };
+  using QP = std::conditional_t<SC, QPScaled, QPFull>;
   static constexpr int KW = (DO_K || DO_ASM) ? T::NPAIRS * NF * NF : 1;  // staged element-matrix words
 };
 
@@ -71,7 +82,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   constexpr int D = NF * ND;
   constexpr bool DO_R = LDS::DO_R, DO_K = LDS::DO_K, DO_MF = LDS::DO_MF, DO_ASM = LDS::DO_ASM, DO_D = LDS::DO_D;
   constexpr bool DO_G = (MODE & MODE_QPGRAD) != 0;
-  constexpr bool NEED_C = LDS::NEED_C, GS = LDS::GS;
+  constexpr bool NEED_C = LDS::NEED_C, GS = LDS::GS, SC = LDS::SC, QR = LDS::QR;
   __shared__ LDS lds[EPW];
   __shared__ __attribute__((aligned(16))) ElemWork<LDS, EPW, NQ> work;
   constexpr int KW = LDS::KW;
@@ -88,10 +99,14 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     if constexpr (GS) {
 #pragma unroll
       for (int c = 0; c < M::NC; ++c) C[c] = Q.C[c];
-    } else {
+    } else if constexpr (M::NS > 0) {
 #pragma unroll
       for (int c = 0; c < M::NS; ++c) C[M::S0 + c] = Q.S[c];
     }
+  };
+  auto qp_weight = [](const auto& Q) {  // the scaled record holds sqrt(w) g: weight 1 in the stiffness
+    if constexpr (SC) return 1.0f;
+    else return Q.w;
   };
   auto node_vec = [&](const auto& Q, const float* C, int b, float* vb) {
     if constexpr (GS) {
@@ -210,7 +225,9 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         g[k][j] = t;
       }
     const float w = det * qweight<KIND>();
-    if (r == 0) Q.w = w;
+    if constexpr (!SC) {
+      if (r == 0) Q.w = w;
+    }
     float H[D] = {};
 #pragma unroll
     for (int k = 0; k < NMY; ++k)
@@ -255,9 +272,11 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     for (int c = 0; c < D; ++c)
       if (!(P[c] == P[c])) bad = true;
     if constexpr (DO_R) {
-      if (r == 0) {
+      if constexpr (!QR) {
+        if (r == 0) {
 #pragma unroll
-        for (int c = 0; c < D; ++c) Q.P[c] = P[c];
+          for (int c = 0; c < D; ++c) Q.P[c] = P[c];
+        }
       }
       if constexpr (PHYS == FECX_POISSON) {
         float f = args.mp.p[1];
@@ -273,7 +292,27 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
             f *= sinpif(L.x0[d] + t);
           }
         }
-        if (r == 0) Q.src[0] = f;
+        if constexpr (QR) {
+          // R_a = sum_q w (P . g_a - N_a f), the node phase's expression, for all 8 nodes of this point; the sums
+          // over the element's 8 points (lanes) by DPP: xor 1 and xor 2 within quads, then the half-row mirror
+          // (lane i <-> 7 - i) adds the other quad. Every lane ends with the same bits; lane a keeps node a.
+          float ra = 0.0f;
+#pragma unroll
+          for (int a = 0; a < NNPE; ++a) {
+            float t = 0;
+#pragma unroll
+            for (int j = 0; j < ND; ++j) t += P[j] * g[a][j];
+            t -= N[a] * f;
+            t *= w;
+            t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0xB1, 0xF, 0xF, false));
+            t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x4E, 0xF, 0xF, false));
+            t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x141, 0xF, 0xF, false));
+            ra = q == a ? t : ra;
+          }
+          args.Rel[e * NDOF + q] = ra;
+        } else {
+          if (r == 0) Q.src[0] = f;
+        }
       }
     }
     if constexpr (GS) {
@@ -284,19 +323,29 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
 #pragma unroll
         for (int j = 0; j < ND; ++j) Q.v[a][j] = g[a][j];
     } else if constexpr (DO_K || DO_ASM || DO_D) {
-      if (r == 0) {
+      if constexpr (M::NS > 0) {
+        if (r == 0) {
 #pragma unroll
-        for (int c = 0; c < M::NS; ++c) Q.S[c] = C[M::S0 + c];
-      }
-#pragma unroll
-      for (int k = 0; k < NMY; ++k)
-        if (mine(k)) {
-          float va[M::NV];
-          M::template prep<ND>(args.mp, C, g[k], va);
-#pragma unroll
-          for (int c = 0; c < M::NV; ++c) Q.v[r + S * k][c] = va[c];
+          for (int c = 0; c < M::NS; ++c) Q.S[c] = C[M::S0 + c];
         }
-    } else {
+      }
+      if constexpr (SC) {
+        const float sw = sqrtf(w);  // K_ab = sum_q (sqrt(w) g_a) . (sqrt(w) g_b)
+#pragma unroll
+        for (int a = 0; a < NNPE; ++a)
+#pragma unroll
+          for (int j = 0; j < ND; ++j) Q.v[a][j] = sw * g[a][j];
+      } else {
+#pragma unroll
+        for (int k = 0; k < NMY; ++k)
+          if (mine(k)) {
+            float va[M::NV];
+            M::template prep<ND>(args.mp, C, g[k], va);
+#pragma unroll
+            for (int c = 0; c < M::NV; ++c) Q.v[r + S * k][c] = va[c];
+          }
+      }
+    } else if constexpr (!SC) {
 #pragma unroll
       for (int k = 0; k < NMY; ++k)
         if (mine(k))
@@ -330,7 +379,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   float kreg[(DO_K || DO_ASM) ? MAXP : 1][NF * NF];
   if (valid && lane < NNPE) {
     const int a = lane;
-    if constexpr (DO_R || DO_MF) {
+    if constexpr ((DO_R || DO_MF) && !QR) {
       float out[NF] = {};
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
@@ -367,7 +416,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         float va[M::NV], sa[M::NA];
         load_c(Q, C);
         node_vec(Q, C, a, va);
-        M::template side<ND>(args.mp, C, Q.w, va, sa);
+        M::template side<ND>(args.mp, C, qp_weight(Q), va, sa);
 #pragma unroll
         for (int p = 0; p < MAXP; ++p) {
           if (p < cnt) {
@@ -412,7 +461,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         float va[M::NV], sa[M::NA];
         load_c(Q, C);
         node_vec(Q, C, a, va);
-        M::template side<ND>(args.mp, C, Q.w, va, sa);
+        M::template side<ND>(args.mp, C, qp_weight(Q), va, sa);
         M::template pair<ND>(sa, va, acc);
       }
 #pragma unroll
