@@ -37,8 +37,9 @@ struct ElemLDS {
   // of 36 KB per work-group: five work-groups per CU instead of four).
   // The element blocks need only sqrt(w) g_a per point and node (24 floats per point); every Poisson hex8 mode that
   // forms them stages that (SC), so fused and separate calls give the same bits.
-  static constexpr bool SC = KIND == FECX_HEX8 && PHYS == FECX_POISSON && !DO_MF && (MODE & MODE_QPGRAD) == 0;
-  static constexpr bool QR = SC && DO_R;
+  // The matrix-free action is summed the same way (QM) and stages nothing.
+  static constexpr bool SC = KIND == FECX_HEX8 && PHYS == FECX_POISSON && (MODE & MODE_QPGRAD) == 0;
+  static constexpr bool QR = SC && DO_R, QM = SC && DO_MF;
   float x[T::NNPE][T::ND];
   float x0[T::ND];
   float u[T::NNPE][NF];
@@ -82,7 +83,15 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   constexpr int D = NF * ND;
   constexpr bool DO_R = LDS::DO_R, DO_K = LDS::DO_K, DO_MF = LDS::DO_MF, DO_ASM = LDS::DO_ASM, DO_D = LDS::DO_D;
   constexpr bool DO_G = (MODE & MODE_QPGRAD) != 0;
-  constexpr bool NEED_C = LDS::NEED_C, GS = LDS::GS, SC = LDS::SC, QR = LDS::QR;
+  constexpr bool NEED_C = LDS::NEED_C, GS = LDS::GS, SC = LDS::SC, QR = LDS::QR, QM = LDS::QM;
+  // sum over the 8 quadrature points (lanes) of a hex8 element: quad xor 1, quad xor 2, then the half-row mirror
+  // (lane i <-> 7 - i) adds the other quad; every lane ends with the same bits
+  auto sum_points8 = [](float t) {
+    t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0xB1, 0xF, 0xF, false));
+    t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x4E, 0xF, 0xF, false));
+    t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x141, 0xF, 0xF, false));
+    return t;
+  };
   __shared__ LDS lds[EPW];
   __shared__ __attribute__((aligned(16))) ElemWork<LDS, EPW, NQ> work;
   constexpr int KW = LDS::KW;
@@ -293,9 +302,8 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
           }
         }
         if constexpr (QR) {
-          // R_a = sum_q w (P . g_a - N_a f), the node phase's expression, for all 8 nodes of this point; the sums
-          // over the element's 8 points (lanes) by DPP: xor 1 and xor 2 within quads, then the half-row mirror
-          // (lane i <-> 7 - i) adds the other quad. Every lane ends with the same bits; lane a keeps node a.
+          // R_a = sum_q w (P . g_a - N_a f), the node phase's expression, for all 8 nodes of this point, summed over
+          // the element's 8 points; lane a keeps node a
           float ra = 0.0f;
 #pragma unroll
           for (int a = 0; a < NNPE; ++a) {
@@ -303,10 +311,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
 #pragma unroll
             for (int j = 0; j < ND; ++j) t += P[j] * g[a][j];
             t -= N[a] * f;
-            t *= w;
-            t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0xB1, 0xF, 0xF, false));
-            t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x4E, 0xF, 0xF, false));
-            t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x141, 0xF, 0xF, false));
+            t = sum_points8(t * w);
             ra = q == a ? t : ra;
           }
           args.Rel[e * NDOF + q] = ra;
@@ -365,9 +370,22 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
       for (int c = 0; c < D; ++c) lane_sum(Gv[c]);
       float out[D];
       M::template dP<ND>(args.mp, C, Gv, out);
-      if (r == 0) {
+      if constexpr (QM) {  // (K v)_a = sum_q w (A : Gv) . g_a, summed over the 8 points like the residual
+        float ra = 0.0f;
 #pragma unroll
-        for (int c = 0; c < D; ++c) Q.P[c] = out[c];
+        for (int a = 0; a < NNPE; ++a) {
+          float t = 0;
+#pragma unroll
+          for (int j = 0; j < ND; ++j) t += out[j] * g[a][j];
+          t = sum_points8(t * w);
+          ra = q == a ? t : ra;
+        }
+        args.Rel[e * NDOF + q] = ra;
+      } else {
+        if (r == 0) {
+#pragma unroll
+          for (int c = 0; c < D; ++c) Q.P[c] = out[c];
+        }
       }
     }
     }  // !DO_G
@@ -379,7 +397,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   float kreg[(DO_K || DO_ASM) ? MAXP : 1][NF * NF];
   if (valid && lane < NNPE) {
     const int a = lane;
-    if constexpr ((DO_R || DO_MF) && !QR) {
+    if constexpr ((DO_R || DO_MF) && !QR && !QM) {
       float out[NF] = {};
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
