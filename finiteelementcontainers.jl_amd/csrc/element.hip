@@ -37,9 +37,13 @@ struct ElemLDS {
   // of 36 KB per work-group: five work-groups per CU instead of four).
   // The element blocks need only sqrt(w) g_a per point and node (24 floats per point); every Poisson hex8 mode that
   // forms them stages that (SC), so fused and separate calls give the same bits.
-  // The matrix-free action is summed the same way (QM) and stages nothing.
-  static constexpr bool SC = KIND == FECX_HEX8 && PHYS == FECX_POISSON && (MODE & MODE_QPGRAD) == 0;
-  static constexpr bool QR = SC && DO_R, QM = SC && DO_MF;
+  // The matrix-free action is summed the same way (QM) and stages nothing. The other hex8 materials sum their residual
+  // and action this way too, in every mode (fused and separate calls give the same bits); where no element block is
+  // formed they stage nothing (NOQP), so LDS no longer limits their occupancy.
+  static constexpr bool HEX = KIND == FECX_HEX8 && (MODE & MODE_QPGRAD) == 0;
+  static constexpr bool SC = HEX && PHYS == FECX_POISSON;
+  static constexpr bool QR = HEX && DO_R, QM = HEX && DO_MF;
+  static constexpr bool NOQP = HEX && !(DO_K || DO_ASM || DO_D);  // no node-phase consumer of quadrature data
   float x[T::NNPE][T::ND];
   float x0[T::ND];
   float u[T::NNPE][NF];
@@ -47,7 +51,7 @@ struct ElemLDS {
   struct QPFull {
     float v[T::NNPE][(DO_K || DO_ASM || DO_D) && !GS ? Mat<PHYS>::NV : T::ND];  // v[a][0..ND) = shape gradient g_a
     float w;
-    float P[(DO_R || DO_MF) ? D : 1];
+    float P[(DO_R && !QR) || (DO_MF && !QM) ? D : 1];
     float src[(DO_R && PHYS == FECX_POISSON) ? 1 : 1];
     union {
       float S[Mat<PHYS>::NS > 0 && !GS ? Mat<PHYS>::NS : 1];  // side scalars of the tangent (C[S0 .. S0+NS))
@@ -57,7 +61,7 @@ struct ElemLDS {
   struct QPScaled {
     float v[T::NNPE][T::ND];  // sqrt(w) g_a
   };
-  using QP = std::conditional_t<SC, QPScaled, QPFull>;
+  using QP = std::conditional_t<SC || NOQP, QPScaled, QPFull>;
   static constexpr int KW = (DO_K || DO_ASM) ? T::NPAIRS * NF * NF : 1;  // staged element-matrix words
 };
 
@@ -74,6 +78,27 @@ struct ElemWork {
   };
 };
 
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+// Reduce-scatter over the 8 quadrature-point lanes of a hex8 element. On entry each lane holds v[a * NF + i], its
+// point's contribution to node a, component i; on exit lane a holds out[i], the sum of those over the 8 points. Three
+// exchanges halve the nodes a lane keeps: with lane 7 - q (half-row mirror; lanes 0-3 keep nodes 0-3), then with
+// q ^ 2 and q ^ 1 (quad permutations): 7 NF exchanges, where one all-reduce per value takes 24 NF.
+template <int NF>
+__device__ __forceinline__ void scatter_points8(const float (&v)[8 * NF], int q, float (&out)[NF]) {
+  const bool h = (q & 4) != 0, b = (q & 2) != 0, c = (q & 1) != 0;
+  float v1[4 * NF], v2[2 * NF];
+#pragma unroll
+  for (int k = 0; k < 4 * NF; ++k) v1[k] = (h ? v[4 * NF + k] : v[k]) + dpp_mov<0x141>(h ? v[k] : v[4 * NF + k]);
+#pragma unroll
+  for (int k = 0; k < 2 * NF; ++k) v2[k] = (b ? v1[2 * NF + k] : v1[k]) + dpp_mov<0x4E>(b ? v1[k] : v1[2 * NF + k]);
+#pragma unroll
+  for (int k = 0; k < NF; ++k) out[k] = (c ? v2[NF + k] : v2[k]) + dpp_mov<0xB1>(c ? v2[k] : v2[NF + k]);
+}
+
 template <int KIND, int NF, int PHYS, int MODE>
 __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   using T = ET<KIND>;
@@ -83,15 +108,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   constexpr int D = NF * ND;
   constexpr bool DO_R = LDS::DO_R, DO_K = LDS::DO_K, DO_MF = LDS::DO_MF, DO_ASM = LDS::DO_ASM, DO_D = LDS::DO_D;
   constexpr bool DO_G = (MODE & MODE_QPGRAD) != 0;
-  constexpr bool NEED_C = LDS::NEED_C, GS = LDS::GS, SC = LDS::SC, QR = LDS::QR, QM = LDS::QM;
-  // sum over the 8 quadrature points (lanes) of a hex8 element: quad xor 1, quad xor 2, then the half-row mirror
-  // (lane i <-> 7 - i) adds the other quad; every lane ends with the same bits
-  auto sum_points8 = [](float t) {
-    t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0xB1, 0xF, 0xF, false));
-    t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x4E, 0xF, 0xF, false));
-    t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x141, 0xF, 0xF, false));
-    return t;
-  };
+  constexpr bool NEED_C = LDS::NEED_C, GS = LDS::GS, SC = LDS::SC, QR = LDS::QR, QM = LDS::QM, NOQP = LDS::NOQP;
   __shared__ LDS lds[EPW];
   __shared__ __attribute__((aligned(16))) ElemWork<LDS, EPW, NQ> work;
   constexpr int KW = LDS::KW;
@@ -234,7 +251,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         g[k][j] = t;
       }
     const float w = det * qweight<KIND>();
-    if constexpr (!SC) {
+    if constexpr (!SC && !NOQP) {
       if (r == 0) Q.w = w;
     }
     float H[D] = {};
@@ -287,8 +304,9 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
           for (int c = 0; c < D; ++c) Q.P[c] = P[c];
         }
       }
+      float f = 0.0f;  // Poisson source at the point
       if constexpr (PHYS == FECX_POISSON) {
-        float f = args.mp.p[1];
+        f = args.mp.p[1];
         if (args.mp.source_kind == 1) {  // the same for every lane of the launch
           f *= (float)ND * 9.8696044010893586f;
 #pragma unroll
@@ -301,23 +319,31 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
             f *= sinpif(L.x0[d] + t);
           }
         }
-        if constexpr (QR) {
-          // R_a = sum_q w (P . g_a - N_a f), the node phase's expression, for all 8 nodes of this point, summed over
-          // the element's 8 points; lane a keeps node a
-          float ra = 0.0f;
-#pragma unroll
-          for (int a = 0; a < NNPE; ++a) {
-            float t = 0;
-#pragma unroll
-            for (int j = 0; j < ND; ++j) t += P[j] * g[a][j];
-            t -= N[a] * f;
-            t = sum_points8(t * w);
-            ra = q == a ? t : ra;
-          }
-          args.Rel[e * NDOF + q] = ra;
-        } else {
+        if constexpr (!QR) {
           if (r == 0) Q.src[0] = f;
         }
+      }
+      if constexpr (QR) {
+        // R_a = sum_q w (P . g_a - N_a f) for the 8 nodes of this point, reduce-scattered over the element's 8
+        // points: lane a keeps node a
+        static_assert(NNPE == 8, "hex8 only");
+        float wP[D], t[NNPE * NF], ra[NF];
+#pragma unroll
+        for (int c = 0; c < D; ++c) wP[c] = w * P[c];
+        const float wf = w * f;
+#pragma unroll
+        for (int a = 0; a < NNPE; ++a)
+#pragma unroll
+          for (int i = 0; i < NF; ++i) {
+            float s = 0;
+#pragma unroll
+            for (int j = 0; j < ND; ++j) s += wP[i * ND + j] * g[a][j];
+            if constexpr (PHYS == FECX_POISSON) s -= N[a] * wf;
+            t[a * NF + i] = s;
+          }
+        scatter_points8<NF>(t, q, ra);
+#pragma unroll
+        for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + q * NF + i] = ra[i];
       }
     }
     if constexpr (GS) {
@@ -350,7 +376,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
             for (int c = 0; c < M::NV; ++c) Q.v[r + S * k][c] = va[c];
           }
       }
-    } else if constexpr (!SC) {
+    } else if constexpr (!SC && !NOQP) {
 #pragma unroll
       for (int k = 0; k < NMY; ++k)
         if (mine(k))
@@ -370,17 +396,22 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
       for (int c = 0; c < D; ++c) lane_sum(Gv[c]);
       float out[D];
       M::template dP<ND>(args.mp, C, Gv, out);
-      if constexpr (QM) {  // (K v)_a = sum_q w (A : Gv) . g_a, summed over the 8 points like the residual
-        float ra = 0.0f;
+      if constexpr (QM) {  // (K v)_a = sum_q w (A : Gv) . g_a, reduce-scattered over the 8 points like the residual
+        float t[NNPE * NF], ra[NF];
 #pragma unroll
-        for (int a = 0; a < NNPE; ++a) {
-          float t = 0;
+        for (int c = 0; c < D; ++c) out[c] *= w;
 #pragma unroll
-          for (int j = 0; j < ND; ++j) t += out[j] * g[a][j];
-          t = sum_points8(t * w);
-          ra = q == a ? t : ra;
-        }
-        args.Rel[e * NDOF + q] = ra;
+        for (int a = 0; a < NNPE; ++a)
+#pragma unroll
+          for (int i = 0; i < NF; ++i) {
+            float s = 0;
+#pragma unroll
+            for (int j = 0; j < ND; ++j) s += out[i * ND + j] * g[a][j];
+            t[a * NF + i] = s;
+          }
+        scatter_points8<NF>(t, q, ra);
+#pragma unroll
+        for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + q * NF + i] = ra[i];
       } else {
         if (r == 0) {
 #pragma unroll
