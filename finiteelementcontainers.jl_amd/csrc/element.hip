@@ -99,6 +99,37 @@ __device__ __forceinline__ void scatter_points8(const float (&v)[8 * NF], int q,
   for (int k = 0; k < NF; ++k) out[k] = (c ? v2[NF + k] : v2[k]) + dpp_mov<0xB1>(c ? v2[k] : v2[NF + k]);
 }
 
+// Element-matrix blocks formed by one lane of the node phase. Every lane forms the blocks K_cb of one centre node c
+// (one evaluation of c's side terms per point). quad4 and hex8: c = lane, b = c + p for the lane's slots
+// (pair_lane_off order). tet10 (16 lanes, 55 blocks): lanes 0-9 form b = c + 0..3; lanes 10-15 form the 15 blocks of
+// offset 4 and 5 around c = lane - 10: K_c,c+4 and K_c,c+5 as stored, K_c,c+6 transposed (its slot holds K_c+6,c,
+// equal by the tangent's major symmetry). The busiest lane forms 4 blocks instead of 6.
+template <int KIND>
+struct PairJobs {
+  static constexpr int NNPE = ET<KIND>::NNPE;
+  static constexpr int LANES = KIND == FECX_TET10 ? 16 : NNPE;  // lanes that form blocks
+  static constexpr int NP = KIND == FECX_TET10 ? 4 : ET<KIND>::MAXP;  // blocks per lane, at most
+  __device__ static int center(int lane) { return lane < NNPE ? lane : lane - NNPE; }
+  // block p of `lane`: false if the lane has none, else the other node b, the storage slot and whether the slot holds
+  // the transpose of the computed K_cb
+  __device__ static bool job(int lane, int p, int& b, int& slot, bool& tr) {
+    const int c = center(lane);
+    if (KIND != FECX_TET10 || lane < NNPE) {
+      const int lim = KIND == FECX_TET10 ? 4 : lane_cnt(NNPE, c);
+      b = c + p < NNPE ? c + p : c + p - NNPE;
+      slot = pair_lane_off(NNPE, c) + p;
+      tr = false;
+      return p < lim;
+    }
+    const int d = p == 0 ? 4 : (p == 1 ? 6 : 5);
+    const int o = p == 1 ? c + 6 : c;  // the node whose slots hold the block
+    b = c + d < NNPE ? c + d : c + d - NNPE;
+    slot = pair_lane_off(NNPE, o) + (p == 2 ? 5 : 4);
+    tr = p == 1;
+    return p == 0 ? c <= 5 : (p == 1 ? c <= 3 : (p == 2 ? c <= 4 : false));
+  }
+};
+
 template <int KIND, int NF, int PHYS, int MODE>
 __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   using T = ET<KIND>;
@@ -424,7 +455,8 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   __syncthreads();
 
   // ---------------- node rows (lane = a)
-  constexpr int MAXP = T::MAXP;
+  using PJ = PairJobs<KIND>;
+  constexpr int MAXP = PJ::NP;
   float kreg[(DO_K || DO_ASM) ? MAXP : 1][NF * NF];
   if (valid && lane < NNPE) {
     const int a = lane;
@@ -451,8 +483,10 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
 #pragma unroll
       for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + a * NF + i] = out[i];
     }
-    if constexpr (DO_K || DO_ASM) {
-      const int cnt = lane_cnt(NNPE, a);
+  }
+  if constexpr (DO_K || DO_ASM) {
+    if (valid && lane < PJ::LANES) {
+      const int a = PJ::center(lane);
       float acc[MAXP][NF * NF];
 #pragma unroll
       for (int p = 0; p < MAXP; ++p)
@@ -468,8 +502,9 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         M::template side<ND>(args.mp, C, qp_weight(Q), va, sa);
 #pragma unroll
         for (int p = 0; p < MAXP; ++p) {
-          if (p < cnt) {
-            const int b = a + p < NNPE ? a + p : a + p - NNPE;
+          int b, slot;
+          bool tr;
+          if (PJ::job(lane, p, b, slot, tr)) {
             float vb[M::NV];
             node_vec(Q, C, b, vb);
             M::template pair<ND>(sa, vb, acc[p]);
@@ -477,16 +512,19 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         }
       }
       if constexpr (NF > 1) {
-        // the diagonal block K_aa of these symmetric tangents is symmetric up to the FP32 rounding of its two FMA
-        // orders; average it with its transpose so that the assembled Jacobian is symmetric bit for bit
+        // the diagonal block K_aa (block 0 of lanes below NNPE) of these symmetric tangents is symmetric up to the
+        // FP32 rounding of its two FMA orders; average it with its transpose so that the assembled Jacobian is
+        // symmetric bit for bit
+        if (lane < NNPE) {
 #pragma unroll
-        for (int i = 0; i < NF; ++i)
+          for (int i = 0; i < NF; ++i)
 #pragma unroll
-          for (int k = i + 1; k < NF; ++k) {
-            const float m = 0.5f * (acc[0][i * NF + k] + acc[0][k * NF + i]);
-            acc[0][i * NF + k] = m;
-            acc[0][k * NF + i] = m;
-          }
+            for (int k = i + 1; k < NF; ++k) {
+              const float m = 0.5f * (acc[0][i * NF + k] + acc[0][k * NF + i]);
+              acc[0][i * NF + k] = m;
+              acc[0][k * NF + i] = m;
+            }
+        }
       }
       if constexpr (DO_K || DO_ASM) {
         // every lane of the work-group has finished reading the quadrature data of its element only after the
@@ -498,6 +536,21 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
       }
     }
   }
+  // the lane's blocks into the element's LDS block storage, each at its slot (transposed where the slot holds K_bc)
+  auto stage_blocks = [&](float* Kblk) {
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) {
+      int b, slot;
+      bool tr;
+      if (PJ::job(lane, p, b, slot, tr)) {
+        float* dst = Kblk + slot * (NF * NF);
+#pragma unroll
+        for (int i = 0; i < NF; ++i)
+#pragma unroll
+          for (int k = 0; k < NF; ++k) dst[i * NF + k] = tr ? kreg[p][k * NF + i] : kreg[p][i * NF + k];
+      }
+    }
+  };
   if constexpr (DO_D) {
     // diagonal of the element Jacobian (the AssembledDiagonal return type): lane a forms its own block K_aa only
     if (valid && lane < NNPE) {
@@ -521,15 +574,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     // element action K_el v_el without atomics: stage the element's symmetric blocks in LDS (over the consumed
     // quadrature data), then lane a sums its row K_ab v_b over b in ascending order
     __syncthreads();
-    if (valid && lane < NNPE) {
-      const int cnt = lane_cnt(NNPE, lane);
-      float* dst = work.K + grp * KW + pair_lane_off(NNPE, lane) * (NF * NF);
-#pragma unroll
-      for (int p = 0; p < MAXP; ++p)
-        if (p < cnt)
-#pragma unroll
-          for (int c = 0; c < NF * NF; ++c) dst[p * NF * NF + c] = kreg[p][c];
-    }
+    if (valid && lane < PJ::LANES) stage_blocks(work.K + grp * KW);
     __syncthreads();
     if (valid && lane < NNPE) {
       float kv[NF] = {};
@@ -549,15 +594,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     // stage the blocks in LDS (over the consumed quadrature data), then the whole wavefront copies its elements'
     // contiguous storage range with consecutive lanes on consecutive words
     __syncthreads();
-    if (valid && lane < NNPE) {
-      const int cnt = lane_cnt(NNPE, lane);
-      float* dst = work.K + grp * KW + pair_lane_off(NNPE, lane) * (NF * NF);
-#pragma unroll
-      for (int p = 0; p < MAXP; ++p)
-        if (p < cnt)
-#pragma unroll
-          for (int c = 0; c < NF * NF; ++c) dst[p * NF * NF + c] = kreg[p][c];
-    }
+    if (valid && lane < PJ::LANES) stage_blocks(work.K + grp * KW);
     __syncthreads();
     constexpr int EW = T::NPAIRS * NF * NF;   // words per element
     constexpr int EPWAVE = 64 / G;            // elements per wavefront
