@@ -11,6 +11,7 @@
 // Three kernels share this contract:
 //   reduce_rows_kernel          slot-owner lanes (vector rows, scalar tet10/quad4 rows, every heavy row);
 //   reduce_scalar_hex8_kernel   scalar hex8 light rows: lanes add their words into LDS slot accumulators instead;
+//   reduce_heavy_scatter_kernel tet10 heavy rows of up to 72 blocks: the same scatter, one wavefront per row;
 //   reduce_vector_kernel / reduce_vector_group_kernel   calls without CSR values (one thread, or 8 lanes, per row).
 // All sum a slot's (or a row's) contributions in ascending incidence order, so every form gives the same bits.
 #include <algorithm>
@@ -313,6 +314,127 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scalar_hex8_kernel(ReduceA
 #define RED_SCATTER 1
 #endif
 
+// Heavy rows (tet10 vertex rows: 24 incidences, 65 blocks) as a scatter, one wavefront per row. Lane l holds words
+// l and l + 64 of the row's staged K_a* (block b = word / NF^2, entry c = word % NF^2) for each incidence, loaded 8
+// incidences at a time, and adds each into the FP64 accumulator of (slot of b, c) in LDS, incidence after incidence
+// in ascending order. Within one incidence the blocks land in distinct slots, and the LDS operations of one wavefront
+// complete in order, so every accumulator sums its words in the slot-owner kernel's order, with the same bits. The
+// slot-owner form tested every staged block's slot byte for every slot (240 tests per lane on a vertex row) and ran
+// the 9 FP64 adds of a block on every lane whenever one lane matched: about 1,300 instructions per wavefront, which
+// bound its time. Rows of more than SMAX blocks keep the slot-owner kernel.
+template <int NF, int NNPE, int SMAX>
+__global__ __launch_bounds__(RED_THREADS) void reduce_heavy_scatter_kernel(ReduceArgs a) {
+  constexpr int LPR = 64, NPW = RED_THREADS / LPR, CHK = RED_HEAVY_CH, LB = RED_LB1;
+  constexpr int NB = NF * NF;                      // words per block
+  constexpr int EW = NNPE * (NNPE + 1) / 2 * NB;   // stored words per element
+  constexpr int VW = (NNPE + 3) / 4;               // vslot words per incidence
+  constexpr int BW = NNPE * NB;                    // staged words per incidence
+  constexpr int JN = (BW + LPR - 1) / LPR;         // words per lane and incidence
+  static_assert(CHK <= LPR && CHK % LB == 0, "a row's lanes stage its incidences, LB per batch");
+  __shared__ int16_t s_off[NNPE][BW];
+  __shared__ int32_t s_v[NPW][CHK];
+  __shared__ uint32_t s_w[NPW][CHK][VW];
+  __shared__ float s_r[NPW][CHK][NF];
+  __shared__ double s_acc[NPW][SMAX * NB];
+  __shared__ int s_maxpass;
+  for (int i = threadIdx.x; i < NNPE * BW; i += RED_THREADS) (&s_off[0][0])[i] = a.kel_off[i];
+  const int64_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
+  const int64_t pos = a.p0 + chunk * NPW + grp;
+  const bool valid = pos < a.p1;
+  const int4 ri = valid ? a.rowinfo[pos] : make_int4(0, 0, 0, 0);
+  const int64_t n = ri.x;
+  const int32_t c0 = ri.y, ninc = ri.w & 0xFFFF;
+  const int deg = a.vals ? (ri.w >> 16) : 0;  // 0: no CSR blocks (a node with Dirichlet dofs)
+  const int npass = (ninc + CHK - 1) / CHK;
+  if (threadIdx.x == 0) s_maxpass = 0;
+  for (int k = lane; k < deg * NB; k += LPR) s_acc[grp][k] = 0.0;
+  __syncthreads();
+  if (lane == 0) atomicMax(&s_maxpass, npass);
+  __syncthreads();
+  const int maxpass = s_maxpass;
+  // this lane's words: block and entry of word jj * 64 + lane
+  int wb[JN], wc[JN];
+#pragma unroll
+  for (int jj = 0; jj < JN; ++jj) {
+    const int j = jj * LPR + lane;
+    wb[jj] = j < BW ? j / NB : 0;
+    wc[jj] = j % NB;
+  }
+  double racc = 0.0;
+  for (int pass = 0; pass < maxpass; ++pass) {
+    const int base = pass * CHK;
+    const int cnt = min(CHK, ninc - base);  // may be <= 0 for this node
+    if (lane < cnt) {
+      const int64_t c = (int64_t)c0 + base + lane;
+      const int32_t v = a.vent[c];
+      s_v[grp][lane] = v;
+      const uint32_t* sl = reinterpret_cast<const uint32_t*>(a.vslot) + c * VW;
+#pragma unroll
+      for (int q = 0; q < VW; ++q) s_w[grp][lane][q] = sl[q];
+      if (a.R) {
+#pragma unroll
+        for (int i = 0; i < NF; ++i) s_r[grp][lane][i] = a.Rel[(int64_t)v * NF + i];
+      }
+    }
+    __syncthreads();
+    if (a.R && lane < NF) {
+      for (int t = 0; t < cnt; ++t) racc += (double)s_r[grp][t][lane];
+    }
+    if (deg > 0) {
+#pragma unroll
+      for (int t0 = 0; t0 < CHK; t0 += LB) {
+        if (t0 >= cnt) break;
+        float w[LB][JN];  // all loads of the batch in flight together
+#pragma unroll
+        for (int tt = 0; tt < LB; ++tt) {
+          const int t = t0 + tt;
+          const uint32_t v = (uint32_t)s_v[grp][t < cnt ? t : 0];
+          const uint32_t ebase = (v / NNPE) * (uint32_t)EW;  // element storage is limited to 2^31 words
+          const int la = (int)(v % NNPE);
+#pragma unroll
+          for (int jj = 0; jj < JN; ++jj) {
+            const int j = jj * LPR + lane;
+            w[tt][jj] = a.Kel[ebase + (j < BW ? (uint32_t)s_off[la][j] : 0u)];
+          }
+        }
+#pragma unroll
+        for (int tt = 0; tt < LB; ++tt) {
+          const int t = t0 + tt;
+          if (t < cnt) {
+#pragma unroll
+            for (int jj = 0; jj < JN; ++jj) {
+              if (jj * LPR + lane < BW) {
+                const int b = wb[jj];
+                const uint32_t sb = (s_w[grp][t][b >> 2] >> (8 * (b & 3))) & 0xFFu;  // the block's slot in the row
+                if (sb != 0xFFu) s_acc[grp][sb * NB + wc[jj]] += (double)w[tt][jj];
+              }
+            }
+          }
+        }
+      }
+    }
+    if (pass + 1 < maxpass) __syncthreads();  // the next pass restages s_v, s_w and s_r
+  }
+  if (valid && a.R && lane < NF) a.R[n * NF + lane] = racc;
+  // one wavefront wrote this row's accumulators, in order: no barrier before reading them back
+  const int64_t rb = ri.z;
+#pragma unroll
+  for (int i = 0; i < NF; ++i) {
+    double* dst = a.vals + (int64_t)NB * rb + (int64_t)i * NF * deg;
+    for (int m = lane; m < NF * deg; m += LPR) {
+      const double x = s_acc[grp][(m / NF) * NB + i * NF + m % NF];
+      if constexpr (NF >= 3) __builtin_nontemporal_store(x, dst + m);  // written once, never read here
+      else dst[m] = x;
+    }
+  }
+}
+
+#ifndef RED_HEAVY_SCATTER
+#define RED_HEAVY_SCATTER 1
+#endif
+constexpr int HEAVY_SCATTER_SMAX = 72;  // structured tet10 vertex rows: 65 blocks
+
 // One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 8 for
 // scalar rows of up to 32 blocks (hex8: 27, in four sweeps; 32 rows per work-group, every lane busy in the gather of
 // the 8-word scalar blocks: 0.46 ms vs 0.575 ms with 16 lanes on Poisson 128^3), 32 for vector rows of up to 32
@@ -377,8 +499,16 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
   } else {
     rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
   }
-  if (rc == 0) rc = launch_lpr<NF, NNPE>(hi, a.maxdeg, s);
-  return rc;
+  if (rc != 0 || hi.p1 <= hi.p0) return rc;
+  if constexpr (RED_HEAVY_SCATTER && NNPE == 10) {
+    if (a.maxdeg <= HEAVY_SCATTER_SMAX) {
+      int64_t grid = (hi.p1 - hi.p0 + RED_THREADS / 64 - 1) / (RED_THREADS / 64);
+      grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
+      reduce_heavy_scatter_kernel<NF, NNPE, HEAVY_SCATTER_SMAX><<<(unsigned)grid, RED_THREADS, 0, s>>>(hi);
+      return 0;
+    }
+  }
+  return launch_lpr<NF, NNPE>(hi, a.maxdeg, s);
 }
 
 // Vector-only reduction (residual, actions, diagonals: no CSR values): one thread per row sums the row's incident

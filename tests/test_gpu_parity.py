@@ -186,8 +186,11 @@ def test_pipelined_chunks_bit_identical(fec, gpu, name):
         assert np.array_equal(R, out[0][0]) and np.array_equal(K, out[0][1])
 
 
+FANS = {"fan_tet10_nh": (32, 165), "fan11_tet10_nh": (11, 60), "fan13_tet10_nh": (13, 70)}  # ring points, blocks
+
+
 @pytest.mark.parametrize("case", ["one_hex8_nh", "one_tet10_poisson", "all_dirichlet", "one_free_node",
-                                  "fan_tet10_nh", "fan11_tet10_nh"])
+                                  "fan_tet10_nh", "fan11_tet10_nh", "fan13_tet10_nh"])
 def test_edge_meshes(fec, orc, gpu, case):
     """Smallest meshes and degenerate DOF sets: one element, every dof constrained (empty system), one unknown; and a
     vertex row wider than one reduction sweep."""
@@ -202,10 +205,10 @@ def test_edge_meshes(fec, orc, gpu, case):
         X = np.vstack([v] + [0.5 * (v[a] + v[b]) for a, b in edges])
         mesh = fec.TetMesh(X, np.arange(1, 11, dtype=np.int64)[None, :])
         wl = W.Workload(case, mesh, fec.Poisson("sines", 1.0), 1, np.cos(np.arange(10.0)))
-    elif case in ("fan_tet10_nh", "fan11_tet10_nh"):
-        # 32 ring points: a 165-block centre row (128 lanes, two sweeps); 11: 60 blocks and 22 incidences (64 lanes,
-        # three staging passes)
-        mesh = fec.fan_tet10_mesh(32 if case == "fan_tet10_nh" else 11)
+    elif case in FANS:
+        # 32 ring points: a 165-block centre row (slot-owner kernel, 128 lanes, two sweeps); 11: 60 blocks and 22
+        # incidences (scatter kernel, one staging pass); 13: 70 blocks and 26 incidences (scatter kernel, two passes)
+        mesh = fec.fan_tet10_mesh(FANS[case][0])
         wl = W.Workload(case, mesh, fec.NeoHookean(1.0, 0.5), 3, 0.02 * np.sin(np.arange(3.0 * mesh.num_nodes)))
     else:
         mesh = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (3, 3, 3))
@@ -230,7 +233,7 @@ def test_edge_meshes(fec, orc, gpu, case):
     assert len(R) == len(unknown) and K.values.numel() == len(vals)
     assert np.array_equal(K.rowptr.cpu().numpy() + 1, rp) and np.array_equal(K.colval.cpu().numpy() + 1, cv)
     if case.startswith("fan"):
-        assert np.diff(rp).max() // 3 == (165 if case == "fan_tet10_nh" else 60)
+        assert np.diff(rp).max() // 3 == FANS[case][1]
     if len(unknown):
         assert rel(R, R_ref) < TOL["R"] and rel(K.values.cpu().numpy(), vals) < TOL["K"]
         assert rel(KV, KVm_ref) < TOL["KV"]
