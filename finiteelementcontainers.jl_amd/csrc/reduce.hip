@@ -9,9 +9,10 @@
 // Vector: the same groups sum the node's incident element rows into the residual field (the atomic scatter of
 // src/assemblers/Assemblers.jl:72-87 restated as an owner-computes gather).
 // Three kernels share this contract:
-//   reduce_rows_kernel          slot-owner lanes (vector rows, scalar tet10/quad4 rows, every heavy row);
+//   reduce_rows_kernel          slot-owner lanes (scalar tet10/quad4 rows, 2-field quad4 rows, heavy rows of more than
+//                               72 blocks and heavy hex8/quad4 rows);
 //   reduce_scalar_hex8_kernel   scalar hex8 light rows: lanes add their words into LDS slot accumulators instead;
-//   reduce_heavy_scatter_kernel tet10 heavy rows of up to 72 blocks: the same scatter, one wavefront per row;
+//   reduce_scatter_kernel       3-field rows (all light ones, tet10 heavy rows of up to 72 blocks): the same scatter;
 //   reduce_vector_kernel / reduce_vector_group_kernel   calls without CSR values (one thread, or 8 lanes, per row).
 // All sum a slot's (or a row's) contributions in ascending incidence order, so every form gives the same bits.
 #include <algorithm>
@@ -314,22 +315,25 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scalar_hex8_kernel(ReduceA
 #define RED_SCATTER 1
 #endif
 
-// Heavy rows (tet10 vertex rows: 24 incidences, 65 blocks) as a scatter, one wavefront per row. Lane l holds words
-// l and l + 64 of the row's staged K_a* (block b = word / NF^2, entry c = word % NF^2) for each incidence, loaded 8
-// incidences at a time, and adds each into the FP64 accumulator of (slot of b, c) in LDS, incidence after incidence
-// in ascending order. Within one incidence the blocks land in distinct slots, and the LDS operations of one wavefront
-// complete in order, so every accumulator sums its words in the slot-owner kernel's order, with the same bits. The
-// slot-owner form tested every staged block's slot byte for every slot (240 tests per lane on a vertex row) and ran
-// the 9 FP64 adds of a block on every lane whenever one lane matched: about 1,300 instructions per wavefront, which
-// bound its time. Rows of more than SMAX blocks keep the slot-owner kernel.
-template <int NF, int NNPE, int SMAX>
-__global__ __launch_bounds__(RED_THREADS) void reduce_heavy_scatter_kernel(ReduceArgs a) {
-  constexpr int LPR = 64, NPW = RED_THREADS / LPR, CHK = RED_HEAVY_CH, LB = RED_LB1;
+// Vector rows as a scatter, LPR lanes per row within one wavefront (light rows: 32 lanes, at most 8 incidences and 32
+// blocks; tet10 vertex rows: 64 lanes, 24 incidences per pass, 65 blocks). Lane l holds words l, l + LPR, ... of the
+// row's staged K_a* (block b = word / NF^2, entry c = word % NF^2) for each incidence, loaded a batch of incidences
+// at a time, and adds each into the FP64 accumulator of (slot of b, c) in LDS, incidence after incidence in ascending
+// order. Within one incidence the blocks land in distinct slots, and the LDS operations of one wavefront complete in
+// order, so every accumulator sums its words in the slot-owner kernel's order, with the same bits. The slot-owner
+// form tests every staged block's slot byte for every slot (240 tests per lane on a tet10 vertex row) and runs a
+// block's 9 FP64 adds on every lane whenever one lane matched: about 1,300 instructions per wavefront on the vertex
+// rows, which bound its time. Rows of more than SMAX blocks keep the slot-owner kernel.
+template <int NF, int NNPE, int LPR, int CHK, int SMAX>
+__global__ __launch_bounds__(RED_THREADS) void reduce_scatter_kernel(ReduceArgs a) {
+  static_assert(LPR <= 64 && 64 % LPR == 0, "a row's lanes lie in one wavefront");
+  constexpr int NPW = RED_THREADS / LPR;
   constexpr int NB = NF * NF;                      // words per block
   constexpr int EW = NNPE * (NNPE + 1) / 2 * NB;   // stored words per element
   constexpr int VW = (NNPE + 3) / 4;               // vslot words per incidence
   constexpr int BW = NNPE * NB;                    // staged words per incidence
   constexpr int JN = (BW + LPR - 1) / LPR;         // words per lane and incidence
+  constexpr int LB = JN <= 2 ? RED_LB1 : RED_LB;   // incidences whose loads are in flight together
   static_assert(CHK <= LPR && CHK % LB == 0, "a row's lanes stage its incidences, LB per batch");
   __shared__ int16_t s_off[NNPE][BW];
   __shared__ int32_t s_v[NPW][CHK];
@@ -346,13 +350,15 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_heavy_scatter_kernel(Reduc
   const int64_t n = ri.x;
   const int32_t c0 = ri.y, ninc = ri.w & 0xFFFF;
   const int deg = a.vals ? (ri.w >> 16) : 0;  // 0: no CSR blocks (a node with Dirichlet dofs)
-  const int npass = (ninc + CHK - 1) / CHK;
-  if (threadIdx.x == 0) s_maxpass = 0;
   for (int k = lane; k < deg * NB; k += LPR) s_acc[grp][k] = 0.0;
-  __syncthreads();
-  if (lane == 0) atomicMax(&s_maxpass, npass);
-  __syncthreads();
-  const int maxpass = s_maxpass;
+  int maxpass = 1;  // light rows (at most CHK incidences) take one pass
+  if (!a.onepass) {
+    if (threadIdx.x == 0) s_maxpass = 0;
+    __syncthreads();
+    if (lane == 0) atomicMax(&s_maxpass, (ninc + CHK - 1) / CHK);
+  }
+  __syncthreads();  // s_off
+  if (!a.onepass) maxpass = s_maxpass;
   // this lane's words: block and entry of word jj * 64 + lane
   int wb[JN], wc[JN];
 #pragma unroll
@@ -433,6 +439,9 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_heavy_scatter_kernel(Reduc
 #ifndef RED_HEAVY_SCATTER
 #define RED_HEAVY_SCATTER 1
 #endif
+#ifndef RED_LIGHT_SCATTER
+#define RED_LIGHT_SCATTER 1
+#endif
 constexpr int HEAVY_SCATTER_SMAX = 72;  // structured tet10 vertex rows: 65 blocks
 
 // One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 8 for
@@ -496,6 +505,14 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
     } else {
       rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
     }
+  } else if constexpr (NF == 3) {
+    if (RED_LIGHT_SCATTER && lo.p1 > lo.p0 && a.maxdeg_light <= 32) {
+      int64_t grid = (lo.p1 - lo.p0 + RED_THREADS / 32 - 1) / (RED_THREADS / 32);
+      grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
+      reduce_scatter_kernel<NF, NNPE, 32, CH, 32><<<(unsigned)grid, RED_THREADS, 0, s>>>(lo);
+    } else {
+      rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
+    }
   } else {
     rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
   }
@@ -504,7 +521,7 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
     if (a.maxdeg <= HEAVY_SCATTER_SMAX) {
       int64_t grid = (hi.p1 - hi.p0 + RED_THREADS / 64 - 1) / (RED_THREADS / 64);
       grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
-      reduce_heavy_scatter_kernel<NF, NNPE, HEAVY_SCATTER_SMAX><<<(unsigned)grid, RED_THREADS, 0, s>>>(hi);
+      reduce_scatter_kernel<NF, NNPE, 64, RED_HEAVY_CH, HEAVY_SCATTER_SMAX><<<(unsigned)grid, RED_THREADS, 0, s>>>(hi);
       return 0;
     }
   }
