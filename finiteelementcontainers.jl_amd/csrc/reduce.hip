@@ -635,12 +635,25 @@ int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s) {
   if (!a.vals) {
     if (!a.R) return 0;
     if (RED_VEC_GROUP && a.nnpe == 10) {
-      constexpr int G = 8;
-      int64_t grid = (a.p1 - a.p0 + 256 / G - 1) / (256 / G);
-      grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
-      if (nf == 1) reduce_vector_group_kernel<1, G, 24><<<(unsigned)grid, 256, 0, s>>>(a);
-      else if (nf == 3) reduce_vector_group_kernel<3, G, 24><<<(unsigned)grid, 256, 0, s>>>(a);
-      else return -1;
+      // light rows (edge midpoints, at most 8 incidences: one batch of loads) one thread each, then the vertex rows
+      // (up to 24 incidences) 8 lanes each
+      ReduceArgs lo = a, hi = a;
+      lo.p1 = std::min(a.p1, std::max(a.p0, a.nlight));
+      hi.p0 = std::max(a.p0, a.nlight);
+      if (nf != 1 && nf != 3) return -1;
+      if (lo.p1 > lo.p0) {
+        int64_t grid = (lo.p1 - lo.p0 + 255) / 256;
+        grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
+        if (nf == 1) reduce_vector_kernel<1><<<(unsigned)grid, 256, 0, s>>>(lo);
+        else reduce_vector_kernel<3><<<(unsigned)grid, 256, 0, s>>>(lo);
+      }
+      if (hi.p1 > hi.p0) {
+        constexpr int G = 8;
+        int64_t grid = (hi.p1 - hi.p0 + 256 / G - 1) / (256 / G);
+        grid = (grid + 7) / 8 * 8;
+        if (nf == 1) reduce_vector_group_kernel<1, G, 24><<<(unsigned)grid, 256, 0, s>>>(hi);
+        else reduce_vector_group_kernel<3, G, 24><<<(unsigned)grid, 256, 0, s>>>(hi);
+      }
       return 0;
     }
     int64_t grid = (a.p1 - a.p0 + 255) / 256;
