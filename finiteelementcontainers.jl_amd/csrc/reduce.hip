@@ -447,9 +447,10 @@ constexpr int HEAVY_SCATTER_SMAX = 72;  // structured tet10 vertex rows: 65 bloc
 // One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 8 for
 // scalar rows of up to 32 blocks (hex8: 27, in four sweeps; 32 rows per work-group, every lane busy in the gather of
 // the 8-word scalar blocks: 0.46 ms vs 0.575 ms with 16 lanes on Poisson 128^3), 32 for vector rows of up to 32
-// blocks, 64 for up to 64, and 128 for up to 128 (tet10 vertex rows: 65 blocks; 128 lanes with one slot each measured
-// 7.59 ms for the tet10 80^3 reduction against 7.95 ms with 64 lanes in two sweeps, whose second sweep is lane 0 alone;
-// staging all 24 incidences in one pass with 8 loads in flight per lane then took it from 7.82 to 7.21 ms).
+// blocks, 64 for up to 64, and 128 for up to 128 (rows of 65 blocks: 128 lanes with one slot each measured 7.59 ms for
+// the tet10 80^3 reduction against 7.95 ms with 64 lanes in two sweeps, whose second sweep is lane 0 alone; staging all
+// 24 incidences in one pass with 8 loads in flight per lane then took it from 7.82 to 7.21 ms). The scatter kernel
+// has since taken over the 3-field light rows and the tet10 vertex rows of up to 72 blocks; wider rows stay here.
 template <int NF, int NNPE>
 static int launch_lpr(const ReduceArgs& a, int maxdeg, hipStream_t s) {
   if (a.p1 <= a.p0) return 0;
