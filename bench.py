@@ -1,13 +1,15 @@
 """Benchmark of the element-assembly hot path on one MI355X.
 
 A step is one residual + CSR Jacobian assembly (the residual and stiffness calls of one Newton iteration,
-src/Solvers.jl:133-141) over the whole mesh, inputs resident in HBM: DOF update, element kernel (FP32), FP64
-reductions into the residual field and the CSR values. Default workload: 3D scalar Poisson on the unit cube,
-128^3 hex8 (2,097,152 elements), f = 1, u ~ U[-1, 1] per node.
+src/Solvers.jl:133-141) over the whole mesh, inputs resident in HBM: DOF update, FP32 element math, FP64 sums into
+the residual field and the CSR values. Default workload: compressible neo-Hookean on the unit cube, 96^3 hex8
+(884,736 elements, 2,738,019 DOFs, 217,238,121 CSR values), lambda = 1, mu = 0.5,
+u = 0.05 sum_k a_k sin(k pi x) sin(k pi y) sin(k pi z).
 
-Prints one JSON line: elements/s, ms/step, the dominant kernel's roofline fraction (HIP-event timing on the
-assembler's stream), relative errors against the FP64 oracle on a bounded sample mesh, and the oracle's CPU
-throughput on that sample (cpu_baseline, kind "port": the C++ FP64 restatement of the reference's CPU algorithm).
+Prints one JSON line: elements/s, ms/step, the step's HBM roofline fraction priced on the compulsory bytes per element
+(HIP-event kernel timing on the assembler's stream), relative errors of every output (residual, CSR values,
+matrix-free and assembled actions) against the FP64 oracle at the full size, and the oracle's CPU throughput on a
+bounded sample (cpu_baseline, kind "port": the C++ FP64 restatement of the reference's CPU algorithm).
 """
 import argparse
 import json
@@ -211,6 +213,19 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def rel_max(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def cpu_share():
+    """Host threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS when that is set (the share
+    of the host a job is given), and the affinity count itself."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    n = min(aff, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else aff
+    return n, aff
+
+
 def oracle_step(orc, wl, pat, nu, split=None):
     """One residual + CSR Jacobian on the CPU with the reference's algorithm: the residual loop, the element stiffness
     storage, then sparse! over the precomputed pattern (built once, as update_dofs! does). `split`, if given, receives
@@ -233,15 +248,9 @@ def oracle_step(orc, wl, pat, nu, split=None):
     return R, vals
 
 
-def sample_checks(fec, orc, name, n_sample, threads):
-    """GPU vs oracle relative errors and the oracle's CPU time on the same sample mesh."""
-    import torch
+def cpu_baseline_leg(fec, orc, name, n_sample, threads):
+    """The oracle's CPU throughput on a bounded sample of the workload (median of 5 steps after 1 warm-up)."""
     wl = fec.workloads.CONFIGS[name](n_sample)
-    asm, p = build_assembler(fec, wl)
-    Uu = torch.from_numpy(wl.U).to(asm.device)
-    fec.assemble_vector_and_stiffness_(asm, Uu, p)
-    R_gpu = asm.residual_storage.cpu().numpy()
-    K_gpu = asm.stiffness_storage.cpu().numpy()
     N = wl.nf * wl.mesh.num_nodes
     unknown, d2u = orc.update_dofs(N, np.zeros(0, np.int64))
     orc.set_threads(threads)
@@ -250,11 +259,57 @@ def sample_checks(fec, orc, name, n_sample, threads):
     oracle_step(orc, wl, pat, len(unknown))  # warm-up
     split = []
     for _ in range(5):
-        R, vals = oracle_step(orc, wl, pat, len(unknown), split)
+        oracle_step(orc, wl, pat, len(unknown), split)
     split = np.array(split)
     t = float(np.median(split.sum(axis=1)))
     parts = {k: float(np.median(split[:, i])) * 1e3 for i, k in enumerate(("residual", "element_stiffness", "sparse_csr"))}
-    return {"R": rel(R_gpu, R), "K": rel(K_gpu, vals)}, wl.mesh.num_elements, t, parts
+    return wl.mesh.num_elements, t, parts
+
+
+def fullsize_checks(fec, orc, torch, wl, asm, p, Uu, threads):
+    """Every output of the workload at its full size against the FP64 oracle on the host: the residual, the CSR
+    values (oracle element storage reduced with sparse!'s sums, row by row), the matrix-free and the assembled Jacobian
+    actions. Relative errors in the 2-norm and the max-norm, the CSR pattern compared bit for bit."""
+    m = wl.mesh
+    kind = {"QUAD4": orc.QUAD4, "HEX8": orc.HEX8, "TET10": orc.TET10}[m.element_type]
+    props = wl.physics.props()
+    st = np.zeros(m.num_elements * 8 * 7) if wl.physics.kind == orc.J2 else None
+    orc.set_threads(threads)
+    out = {}
+    t0 = time.time()
+    fec.assemble_vector_and_stiffness_(asm, Uu, p)
+    R_gpu = asm.residual_storage.cpu().numpy()
+    K_gpu = asm.stiffness_storage.cpu().numpy()
+    R = orc.assemble_residual(kind, m.coords, m.conn, wl.nf, wl.physics.kind, props, wl.U, st,
+                              None if st is None else np.zeros_like(st))
+    out["residual"] = {"l2": rel(R_gpu, R), "max": rel_max(R_gpu, R)}
+    del R
+    nu = len(asm.dof.unknown_dofs)
+    _, d2u = orc.update_dofs(wl.nf * m.num_nodes, asm.dof.dirichlet_dofs)
+    storage = orc.assemble_stiffness_storage(kind, m.coords, m.conn, wl.nf, wl.physics.kind, props, wl.U, st)
+    rp, cv, vals = orc.sparse_csr_rows(wl.nf, m.conn, d2u, nu, storage)
+    del storage
+    rp_g, cv_g = (t.cpu().numpy() + 1 for t in asm.pattern())
+    out["pattern_bit_exact"] = bool(np.array_equal(rp_g, rp) and np.array_equal(cv_g, cv))
+    del rp, cv, rp_g, cv_g
+    out["jacobian"] = {"l2": rel(K_gpu, vals), "max": rel_max(K_gpu, vals)}
+    del vals, K_gpu
+    log(f"[bench] full-size check: residual and Jacobian done ({time.time() - t0:.1f}s)")
+    N = wl.nf * m.num_nodes
+    Vf = np.cos(np.arange(N, dtype=np.float64))
+    Vu = torch.from_numpy(Vf[asm.dof.unknown_dofs - 1]).to(asm.device)
+    fec.assemble_matrix_free_action_(asm, fec.stiffness_action, Uu, Vu, p)
+    mf_gpu = asm.stiffness_action_storage.cpu().numpy()
+    fec.assemble_matrix_action_(asm, fec.stiffness, Uu, Vu, p)
+    asm_gpu = asm.stiffness_action_storage.cpu().numpy()
+    for key, g, mf in (("matrix_free_action", mf_gpu, True), ("assembled_action", asm_gpu, False)):
+        ref = orc.assemble_action(kind, m.coords, m.conn, wl.nf, wl.physics.kind, props, wl.U, Vf, st, matrix_free=mf)
+        out[key] = {"l2": rel(g, ref), "max": rel_max(g, ref)}
+    out["against"] = (f"FP64 oracle (C++ restatement of the reference algorithm) at the full size, "
+                      f"{m.num_elements} elements; V = cos(dof index) for the actions")
+    log(f"[bench] full-size check done ({time.time() - t0:.1f}s): " + ", ".join(
+        f"{k} {v['l2']:.2e}/{v['max']:.2e}" for k, v in out.items() if isinstance(v, dict)))
+    return out
 
 
 def main():
@@ -262,12 +317,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="poisson_hex8", choices=sorted(SIZES))
+    ap.add_argument("--workload", default="neohookean_hex8", choices=sorted(SIZES))
     ap.add_argument("--n", type=int, default=None, help="elements per direction (default: the config's size)")
     ap.add_argument("--chunks", type=int, default=0, help="pipelined pieces per matrix assembly (0 or 1: one piece)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-check", action="store_true", help="skip the sample parity/CPU legs (profiling runs)")
+    ap.add_argument("--no-check", action="store_true", help="skip the full-size parity and CPU legs (profiling runs)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg (J2, tet10 NH)")
+    ap.add_argument("--no-operators", action="store_true", help="skip the timings of the other entry points")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -282,8 +338,8 @@ def main():
         print(json.dumps({"calibrate_copy_bytes_read": n * 8, "calibrate_copy_bytes_written": n * 8}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU assembly is a design in "
-              "designed (element partition, halo exchange of partial rows) but not a measured path", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): assembly across GPUs (element "
+              "partition, reverse halo exchange of partial rows) is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -309,7 +365,7 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     log(f"[bench] {dt * 1e3:.3f} ms/step")
 
-    # per-kernel durations with HIP events on the assembler's stream
+    # per-kernel durations with HIP events on the stream each kernel runs on
     fec.reset_profile(asm)
     fec.set_profiling(asm, True)
     n_prof = max(3, min(args.steps, 10))
@@ -324,11 +380,19 @@ def main():
     for k, v in kernels.items():
         if v["bytes"]:
             v["gbs"] = v["bytes"] / (v["ms_per_step"] * 1e-3) / 1e9
-    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-    ach = kernels[dom]["gbs"]
-    log("[bench] kernels: " + ", ".join(f"{k} {v['ms_per_step']:.3f} ms ({v['launches_per_step']:.0f} launches) {v.get('gbs', 0):.0f} GB/s" for k, v in kernels.items()))
-
-    traffic = pmc_traffic(name, dom) if n == n_full else None
+        tr = pmc_traffic(name, k) if n == n_full else None
+        if tr:
+            v["traffic"] = tr
+    log("[bench] kernels: " + ", ".join(f"{k} {v['ms_per_step']:.3f} ms ({v['launches_per_step']:.0f} launches)"
+                                        for k, v in kernels.items()))
+    # The roofline of the step's assembly kernels together (the DOF update excluded), priced on the compulsory bytes
+    # per element of residual + tangent: int32 connectivity, FP32 coordinates, FP64 u, FP64 outputs written once, the
+    # int32 COO->CSR map (NDOF^2 per element) and FP32 state.
+    kern_ms = sum(v["ms_per_step"] for k, v in kernels.items() if k in ("element", "reduce", "fused"))
+    bpe = STEP_BYTES.get(name)
+    traffic = None
+    if n == n_full and all("traffic" in v for k, v in kernels.items() if k in ("element", "reduce", "fused")):
+        traffic = sum(v["traffic"] for k, v in kernels.items() if k in ("element", "reduce", "fused"))
     out = {
         "metric": METRIC,
         "value": wl.num_elements / dt,
@@ -345,16 +409,21 @@ def main():
         "config": {"workload": f"{name} {n}^3" if wl.mesh.num_dims == 3 else f"{name} {n}^2",
                    "elements": int(wl.num_elements), "nnz": int(asm.nnz), "step": "residual + CSR Jacobian",
                    "element_math": "fp32", "reductions": "fp64", "chunks": max(1, args.chunks)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic},
         "kernels": kernels,
     }
-    if name in STEP_BYTES:
-        eff = STEP_BYTES[name] * wl.num_elements / dt / 1e9
-        out["step_roofline"] = {"bytes_per_element": STEP_BYTES[name], "achieved": eff, "frac": eff / HBM_PEAK_GBS}
-        out["operators"] = operator_timings(fec, torch, name, wl, asm, p, Uu)
+    if bpe:
+        ach = bpe * wl.num_elements / (kern_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": "+".join(k for k in kernels if k in ("element", "reduce", "fused")),
+                           "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                           "traffic": traffic, "bytes_per_element": bpe, "kernel_ms_per_step": kern_ms}
+        eff = bpe * wl.num_elements / dt / 1e9
+        out["step_roofline"] = {"bytes_per_element": bpe, "achieved": eff, "frac": eff / HBM_PEAK_GBS,
+                                "basis": "wall time per step (DOF update and host synchronisation included)"}
+        if not args.no_operators:
+            out["operators"] = operator_timings(fec, torch, name, wl, asm, p, Uu)
 
     if name in MLP_FLOPS and not args.no_surrogate:
+        fec.assemble_vector_and_stiffness_(asm, Uu, p)
         R_plain = fec.residual(asm).cpu().numpy().copy()
         K_plain = asm.stiffness_storage.cpu().numpy().copy()
         out["surrogate"] = surrogate_leg(fec, torch, name, wl, asm, R_plain, K_plain, args.steps)
@@ -362,15 +431,15 @@ def main():
 
     if not args.no_check:
         orc = ge.load_oracle()
-        threads = min(16, os.cpu_count() or 1)
-        errs, ne_s, t_cpu, parts = sample_checks(fec, orc, name, n_sample, threads)
-        out["rel_err"] = {"residual": errs["R"], "jacobian": errs["K"],
-                          "sample": f"{name} n={n_sample} ({ne_s} elements) vs FP64 oracle"}
+        threads, aff = cpu_share()
+        out["rel_err"] = fullsize_checks(fec, orc, torch, wl, asm, p, Uu, threads)
         if not args.no_cpu_baseline:
+            ne_s, t_cpu, parts = cpu_baseline_leg(fec, orc, name, n_sample, threads)
             out["cpu_baseline"] = {"value": ne_s / t_cpu, "unit": "elements/s", "cores": threads, "kind": "port",
                                    "sample": f"{name} n={n_sample}: {ne_s} elements, residual + element stiffness + "
                                              f"sparse! COO->CSR over the pattern built at setup, median of 5 after "
-                                             f"1 warm-up (C++ restatement of the reference algorithm, OpenMP)",
+                                             f"1 warm-up (C++ FP64 restatement of the reference algorithm, OpenMP on "
+                                             f"{threads} threads: the host share given to this job, affinity {aff})",
                                    "split_ms": parts}
     print(json.dumps(out), flush=True)
 

@@ -124,19 +124,26 @@ def lumped_mass(obj, *args):
     """Callback tag of the row-sum lumped mass; lumped_mass(asm) is the assembled vector over the unknowns
     (src/assemblers/LumpedMass.jl:69-80). It shares storage with residual(asm)."""
     if isinstance(obj, SparseMatrixAssembler):
-        return obj._residual()
+        return obj._full_or_unknowns()
     raise TypeError("lumped_mass(physics, ...) is evaluated inside libfecx.so; pass `lumped_mass` as the callback")
 
 
 def diagonal(asm):
-    """The assembled diagonal over the unknowns (src/assemblers/Diagonal.jl:70-83); shares storage with residual(asm)."""
-    return asm._residual()
+    """The assembled diagonal over the unknowns, or the full vector in condensed mode (src/assemblers/Diagonal.jl:70-83);
+    shares storage with residual(asm)."""
+    return asm._full_or_unknowns()
 
 
 def density(physics):
     """The density the test physics' mass callbacks multiply by: props[1] of Mechanics
     (test/mechanics/TestMechanicsCommon.jl:52-124), 1 for Poisson (test/poisson/TestPoissonCommon.jl:18-41), and the
-    `density` attribute (default 1) of the other materials."""
+    `density` attribute (default 1) of the other materials. With per-block physics every block must have the same
+    density (the mass assemblies take one)."""
+    if isinstance(physics, dict):
+        ds = {density(ph) for ph in physics.values()}
+        if len(ds) != 1:
+            raise FECXError(-4, "mass assemblies take one density; the blocks' physics differ in density")
+        return ds.pop()
     if isinstance(physics, Mechanics):
         return float(physics.props()[0])
     return float(getattr(physics, "density", 1.0))
@@ -238,8 +245,38 @@ class CSRMatrix:
         return torch.sparse_csr_tensor(self.rowptr, self.colval, self.values, size=self.shape)
 
 
+class CSCMatrix:
+    """CSC matrix on the GPU (sparse_matrix_type = :csc, the reference's default): colptr/rowval 0-based int64
+    tensors, nzval float64 (SparseMatrixCSC uses 1-based indices). The element pattern is structurally symmetric and
+    the assembled tangents and masses are symmetric bit for bit, so colptr/rowval/nzval are the CSR rowptr/colval/values
+    of the same matrix, shared without a copy."""
+
+    def __init__(self, n, colptr, rowval, nzval):
+        self.shape = (n, n)
+        self.colptr, self.rowval, self.nzval = colptr, rowval, nzval
+
+    @property
+    def values(self):
+        return self.nzval
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csc_matrix((self.nzval.cpu().numpy(), self.rowval.cpu().numpy(), self.colptr.cpu().numpy()),
+                             shape=self.shape)
+
+
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _set_mesh(ctx, mesh, elem_kind):
+    """fecx_set_mesh_blocks with the mesh's element blocks (one block for meshes without ``block_conns``)."""
+    coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int64)
+    sizes = np.asarray(getattr(mesh, "block_sizes", [conn.shape[0]]), dtype=np.int64)
+    ctx.call("fecx_set_mesh_blocks", ctypes.c_int(elem_kind), ctypes.c_int(coords.shape[1]),
+             ctypes.c_int64(coords.shape[0]), _host_ptr(coords), ctypes.c_int(len(sizes)), _host_ptr(sizes),
+             _host_ptr(conn))
 
 
 def _host_ptr(a):
@@ -247,28 +284,40 @@ def _host_ptr(a):
 
 
 class SparseMatrixAssembler:
-    """SparseMatrixAssembler (src/assemblers/SparseMatrixAssembler.jl:7-133) with one libfecx context."""
+    """SparseMatrixAssembler (src/assemblers/SparseMatrixAssembler.jl:7-133) with one libfecx context.
 
-    def __init__(self, var, sparse_matrix_type="csr", device=0, matrix_free=False):
-        if sparse_matrix_type != "csr":
-            raise FECXError(-4, f"Unsupported sparse matrix type {sparse_matrix_type}. Only :csr is assembled on "
-                                "the GPU.")
+    sparse_matrix_type "csr" or "csc" (the reference's default, SparseMatrixAssembler.jl:66); use_condensed keeps every
+    DOF in the pattern and adjusts the results for the constraints in the accessors (Utils.jl:53-167)."""
+
+    def __init__(self, var, sparse_matrix_type="csr", device=0, matrix_free=False, use_condensed=False,
+                 penalty_scale=1.0e6):
+        if sparse_matrix_type not in ("csr", "csc"):
+            raise FECXError(-4, f"Unsupported sparse matrix type {sparse_matrix_type}. :csr and :csc are assembled "
+                                "on the GPU.")
         if not torch.cuda.is_available():
             raise FECXError(-3, "no GPU: libfecx.so assembles on a HIP device only")
         self.device = torch.device("cuda", device)
+        self.sparse_matrix_type = sparse_matrix_type
         self.matrix_free = matrix_free
+        self.use_condensed = bool(use_condensed)
+        self.penalty_scale = float(penalty_scale)
         self.var = var
         self.dof = DofManager(var)
         mesh = var.fspace.mesh
         self.ctx = _lib.Context(device)
         self.ctx.call("fecx_set_stream", ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
-        coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
-        conn = np.ascontiguousarray(mesh.conn, dtype=np.int64)
-        self.ctx.call("fecx_set_mesh", ctypes.c_int(var.fspace.elem_kind), ctypes.c_int(coords.shape[1]),
-                      ctypes.c_int64(coords.shape[0]), _host_ptr(coords), ctypes.c_int64(conn.shape[0]),
-                      _host_ptr(conn))
+        _set_mesh(self.ctx, mesh, var.fspace.elem_kind)
+        if self.use_condensed:
+            self.ctx.call("fecx_set_condensed", ctypes.c_int(1))
         self._pattern = None
+        self._k_adjusted = self._m_adjusted = True
         self.update_dofs(np.zeros(0, dtype=np.int64))
+
+    @property
+    def n_unknowns(self):
+        """Length of the unknown vectors: the free dofs, or every dof in condensed mode (create_unknowns,
+        src/DofManagers.jl:168-175)."""
+        return len(self.dof) if self.use_condensed else len(self.dof.unknown_dofs)
 
     # update_dofs!(asm, dbcs) (src/assemblers/SparseMatrixAssembler.jl:228-274)
     def update_dofs(self, dirichlet_dofs):
@@ -284,6 +333,7 @@ class SparseMatrixAssembler:
         self.ctx.call("fecx_build_pattern", ctypes.byref(nnz))
         self.nnz = nnz.value
         self._pattern = None
+        self._k_adjusted = self._m_adjusted = True
         dev, N, nu = self.device, len(self.dof), len(unk)
         f64 = torch.float64
         self.residual_storage = torch.zeros(N, dtype=f64, device=dev)
@@ -294,9 +344,10 @@ class SparseMatrixAssembler:
         self.mass_storage = torch.zeros(0 if self.matrix_free else self.nnz, dtype=f64, device=dev)
 
     def pattern(self):
-        """(rowptr, colval) 0-based int64 tensors on the GPU, bit-exact with sparse! + SparseMatrixCSR."""
+        """(rowptr, colval) 0-based int64 tensors on the GPU, bit-exact with sparse! + SparseMatrixCSR (and the
+        (colptr, rowval) of sparse!'s CSC: the pattern is structurally symmetric)."""
         if self._pattern is None:
-            nu = len(self.dof.unknown_dofs)
+            nu = self.n_unknowns
             rp = np.zeros(nu + 1, dtype=np.int64)
             cv = np.zeros(self.nnz, dtype=np.int64)
             self.ctx.call("fecx_get_pattern", _host_ptr(rp), _host_ptr(cv), ctypes.c_int(0))
@@ -311,33 +362,79 @@ class SparseMatrixAssembler:
         return m
 
     def _set_material(self, p):
+        """The physics of every block: one Physics for all blocks, or {block name: Physics} (the reference's
+        per-block physics NamedTuple, src/Parameters.jl:37-73, handed out by foreach_block, src/Utils.jl:171-189)."""
+        if isinstance(p.physics, dict):
+            names = list(getattr(p.mesh, "block_names", ["block_1"]))
+            if sorted(p.physics) != sorted(names):
+                raise FECXError(-1, f"physics given for blocks {sorted(p.physics)}, the mesh has {names}")
+            for b, name in enumerate(names):
+                ph = p.physics[name]
+                props = np.ascontiguousarray(ph.props(), dtype=np.float64)
+                self.ctx.call("fecx_set_block_material", ctypes.c_int(b), ctypes.c_int(ph.kind), _host_ptr(props),
+                              ctypes.c_int(len(props)))
+            return
         props = np.ascontiguousarray(p.physics.props(), dtype=np.float64)
         self.ctx.call("fecx_set_material", ctypes.c_int(p.physics.kind), _host_ptr(props), ctypes.c_int(len(props)))
 
     def _residual(self):
+        """residual(asm) (src/assemblers/Assemblers.jl:347-371): the free-dof subset, or in condensed mode the full
+        vector with its constrained entries zeroed."""
+        if self.use_condensed:
+            self.ctx.call("fecx_adjust_vector_for_constraints", _ptr(self.residual_storage))
+            return self.residual_storage
         self.ctx.call("fecx_extract_field_unknowns", _ptr(self.residual_storage), _ptr(self.residual_unknowns))
         return self.residual_unknowns
 
+    def _full_or_unknowns(self):
+        """The assembled diagonal / lumped mass: full vector in condensed mode (Diagonal.jl:70-83, LumpedMass.jl:69-80)."""
+        if self.use_condensed:
+            return self.residual_storage
+        self.ctx.call("fecx_extract_field_unknowns", _ptr(self.residual_storage), _ptr(self.residual_unknowns))
+        return self.residual_unknowns
+
+    def _matrix(self, values):
+        rp, cv = self.pattern()
+        n = self.n_unknowns
+        if self.sparse_matrix_type == "csc":
+            return CSCMatrix(n, rp, cv, values)
+        return CSRMatrix(n, rp, cv, values)
+
     def _stiffness(self):
+        """stiffness(asm) (Assemblers.jl:376-388); condensed mode adjusts it for the constraints once per assembly."""
         if self.matrix_free:
             raise FECXError(-2, "stiffness is not available on a matrix-free assembler")
-        rp, cv = self.pattern()
-        return CSRMatrix(len(self.dof.unknown_dofs), rp, cv, self.stiffness_storage)
+        if self.use_condensed and not self._k_adjusted:
+            self.ctx.call("fecx_adjust_matrix_for_constraints", _ptr(self.stiffness_storage),
+                          ctypes.c_double(self.penalty_scale))
+            self._k_adjusted = True
+        return self._matrix(self.stiffness_storage)
 
     def _mass(self):
+        """mass(asm) (Assemblers.jl:329-336), adjusted like the stiffness in condensed mode."""
         if self.matrix_free:
             raise FECXError(-2, "mass is not available on a matrix-free assembler")
-        rp, cv = self.pattern()
-        return CSRMatrix(len(self.dof.unknown_dofs), rp, cv, self.mass_storage)
+        if self.use_condensed and not self._m_adjusted:
+            self.ctx.call("fecx_adjust_matrix_for_constraints", _ptr(self.mass_storage),
+                          ctypes.c_double(self.penalty_scale))
+            self._m_adjusted = True
+        return self._matrix(self.mass_storage)
 
     def hvp(self, v):
+        """hvp(asm, v) (Assemblers.jl:310-324): the free-dof subset of the action, or in condensed mode the full
+        action with Av = (1 - c) Av + c v."""
+        if self.use_condensed:
+            _check_vec(self, v, "v")
+            self.ctx.call("fecx_adjust_action_for_constraints", _ptr(self.stiffness_action_storage), _ptr(v))
+            return self.stiffness_action_storage
         self.ctx.call("fecx_extract_field_unknowns", _ptr(self.stiffness_action_storage),
                       _ptr(self.stiffness_action_unknowns))
         return self.stiffness_action_unknowns
 
 
 def create_unknowns(asm):
-    return torch.zeros(len(asm.dof.unknown_dofs), dtype=torch.float64, device=asm.device)
+    """create_unknowns (src/DofManagers.jl:168-175): the free dofs, or every dof in condensed mode."""
+    return torch.zeros(asm.n_unknowns, dtype=torch.float64, device=asm.device)
 
 
 def create_field(asm):
@@ -383,11 +480,13 @@ def _update_for_assembly(p, asm, Uu, Vu=None):
         asm.ctx.call("fecx_update_field_unknowns", _ptr(Vu), _ptr(p.hvp_scratch_field))
 
 
-def _check_vec(asm, x, name):
-    if x.dtype != torch.float64 or x.device != asm.device or not x.is_contiguous():
+def _check_vec(asm, x, name, n=None):
+    """The reference's @assert on vector sizes and backends (src/DofManagers.jl:399-408)."""
+    n = asm.n_unknowns if n is None else n
+    if not isinstance(x, torch.Tensor) or x.dtype != torch.float64 or x.device != asm.device or not x.is_contiguous():
         raise FECXError(-1, f"{name} must be a contiguous float64 tensor on {asm.device}")
-    if x.numel() != len(asm.dof.unknown_dofs):
-        raise FECXError(-1, f"{name} has {x.numel()} entries, expected {len(asm.dof.unknown_dofs)}")
+    if x.numel() != n:
+        raise FECXError(-1, f"{name} has {x.numel()} entries, expected {n}")
 
 
 def assemble_vector_(asm, func, Uu, p):
@@ -404,6 +503,7 @@ def assemble_stiffness_(asm, func, Uu, p):
     _check_vec(asm, Uu, "Uu")
     _update_for_assembly(p, asm, Uu)
     asm.ctx.call("fecx_assemble_matrix", _ptr(p.field), _ptr(asm.stiffness_storage))
+    asm._k_adjusted = False
 
 
 assemble_matrix_ = assemble_stiffness_
@@ -415,6 +515,7 @@ def assemble_vector_and_stiffness_(asm, Uu, p):
     _update_for_assembly(p, asm, Uu)
     asm.ctx.call("fecx_assemble_vector_matrix", _ptr(p.field), _ptr(asm.residual_storage),
                  _ptr(asm.stiffness_storage))
+    asm._k_adjusted = False
 
 
 def assemble_matrix_action_(asm, func, Uu, Vu, p):
@@ -438,6 +539,7 @@ def assemble_mass_(asm, func, Uu, p):
     _check_vec(asm, Uu, "Uu")
     _update_for_assembly(p, asm, Uu)
     asm.ctx.call("fecx_assemble_mass", ctypes.c_double(density(p.physics)), _ptr(asm.mass_storage))
+    asm._m_adjusted = False
 
 
 def assemble_lumped_mass_(asm, func, Uu, p):
@@ -469,10 +571,11 @@ def assemble_matrix_free_action_(asm, func_action, Uu, Vu, p):
 
 def assemble_matrix_free_action_full_(asm, func_action, U_full, v_full, p):
     """assemble_matrix_free_action_full! (src/assemblers/MatrixAction.jl:99-149): caller-built full vectors; the
-    scratch field's Dirichlet slots are reset to zero afterwards."""
+    scratch field's Dirichlet slots are reset to zero afterwards. The full action stays in stiffness_action_storage
+    (no extraction of the unknowns)."""
     N = len(asm.dof)
-    if U_full.numel() != N or v_full.numel() != N:
-        raise FECXError(-1, "full-DOF vectors must have NF*NN entries")
+    _check_vec(asm, U_full, "U_full", N)
+    _check_vec(asm, v_full, "v_full", N)
     p.field.copy_(U_full)
     p.hvp_scratch_field.copy_(v_full)
     asm.ctx.call("fecx_assemble_matrix_free_action", _ptr(p.field), _ptr(p.hvp_scratch_field),
@@ -517,14 +620,13 @@ class HostPattern:
     """libfecx's host pattern builder on a host-only context (device -1): DOF numbering, the reduced CSR pattern
     and the COO->CSR index map, without a GPU. Same code path as SparseMatrixAssembler.update_dofs."""
 
-    def __init__(self, mesh, nf, dirichlet_dofs=()):
+    def __init__(self, mesh, nf, dirichlet_dofs=(), condensed=False):
         self.ctx = _lib.Context(-1)
-        coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
-        conn = np.ascontiguousarray(mesh.conn, dtype=np.int64)
-        self.ne, self.nnpe = conn.shape
+        self.ne, self.nnpe = mesh.conn.shape
         self.nf = nf
-        self.ctx.call("fecx_set_mesh", ctypes.c_int(_ELEM_KIND[mesh.element_type]), ctypes.c_int(coords.shape[1]),
-                      ctypes.c_int64(coords.shape[0]), _host_ptr(coords), ctypes.c_int64(self.ne), _host_ptr(conn))
+        _set_mesh(self.ctx, mesh, _ELEM_KIND[mesh.element_type])
+        if condensed:
+            self.ctx.call("fecx_set_condensed", ctypes.c_int(1))
         d = np.ascontiguousarray(dirichlet_dofs, dtype=np.int64)
         self.ctx.call("fecx_set_dofs", ctypes.c_int(nf), _host_ptr(d) if len(d) else None, ctypes.c_int64(len(d)))
         n = ctypes.c_int64(0)
@@ -535,9 +637,15 @@ class HostPattern:
         nnz = ctypes.c_int64(0)
         self.ctx.call("fecx_build_pattern", ctypes.byref(nnz))
         self.nnz = nnz.value
+        self._nrows = nf * mesh.num_nodes if condensed else len(self.unknown_dofs)
+
+    @property
+    def nrows(self):
+        """Rows of the pattern: the free dofs, or every dof in condensed mode."""
+        return self._nrows
 
     def pattern(self, base=1):
-        rp = np.zeros(len(self.unknown_dofs) + 1, dtype=np.int64)
+        rp = np.zeros(self.nrows + 1, dtype=np.int64)
         cv = np.zeros(self.nnz, dtype=np.int64)
         self.ctx.call("fecx_get_pattern", _host_ptr(rp), _host_ptr(cv), ctypes.c_int(base))
         return rp, cv

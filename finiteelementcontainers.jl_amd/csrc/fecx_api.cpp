@@ -136,8 +136,7 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   const bool surrogate_phys = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
   const bool surrogate = surrogate_phys && ctx->sg_enabled;
   if (surrogate_phys && !surrogate) ea.mp.phys = ctx->mp.phys == FECX_J2_SURROGATE ? FECX_J2 : FECX_NEOHOOKEAN;
-  if (ctx->mp.phys == FECX_J2 || ctx->mp.phys == FECX_J2_SURROGATE)
-    require(ea.st_old != nullptr, FECX_ERR_STATE, "state not allocated");
+  if (ctx->ns > 0) require(ea.st_old != nullptr, FECX_ERR_STATE, "state not allocated");
   if (surrogate) {
     // quadrature-point gradients, then the batched surrogate material pass (network, check, plain fallback)
     require(ctx->sg_arch == (ctx->mp.phys == FECX_J2_SURROGATE ? FECX_MLP_J2 : FECX_MLP_NH), FECX_ERR_STATE,
@@ -155,7 +154,8 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   fecx::ReduceArgs ra = reduce_args(ctx);
   if (mode & (fecx::MODE_R | fecx::MODE_ACTION_MF | fecx::MODE_ACTION_ASM | fecx::MODE_DIAG)) ra.R = R;
   if (mode & fecx::MODE_K) ra.vals = vals;
-  const int k = pipeline_chunks(ctx, mode);
+  const int nblocks = (int)ctx->block_mp.size();
+  const int k = nblocks > 1 ? 1 : pipeline_chunks(ctx, mode);
   hipStream_t es = ctx->stream;
   if (k > 1) {
     if (!ctx->stream2) HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
@@ -170,10 +170,20 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   }
   for (int j = 0; j < k; ++j) {
     const int64_t p0 = ctx->nn * j / k, p1 = ctx->nn * (j + 1) / k;
-    ea.ebeg = ctx->hp.ekey[p0];
-    ea.ne = ctx->hp.ekey[p1];
     int rc = 0;
-    prof.run("element", es, [&] { rc = fecx::launch_element(ctx->kind, ctx->nf, mode, ea, es); });
+    if (nblocks == 1) {
+      ea.ebeg = ctx->hp.ekey[p0];
+      ea.ne = ctx->hp.ekey[p1];
+      prof.run("element", es, [&] { rc = fecx::launch_element(ctx->kind, ctx->nf, mode, ea, es); });
+    } else {  // foreach_block (src/Utils.jl:171-189): each block's storage range with its own material
+      for (int b = 0; b < nblocks && rc == 0; ++b) {
+        fecx::ElementArgs eb = ea;
+        eb.ebeg = ctx->hp.block_pos[b];
+        eb.ne = ctx->hp.block_pos[b + 1];
+        eb.mp = ctx->block_mp[b];
+        prof.run("element", es, [&] { rc = fecx::launch_element(ctx->kind, ctx->nf, mode, eb, es); });
+      }
+    }
     require(rc == 0, FECX_ERR_UNSUPPORTED, "no element kernel for this element kind / nf / physics");
     HIPCHK(hipGetLastError());
     if (k > 1) {
@@ -257,10 +267,22 @@ int fecx_set_stream(fecx_ctx* ctx, void* s) {
 
 int fecx_set_mesh(fecx_ctx* ctx, int elem_kind, int nd, int64_t nn, const double* coords, int64_t ne,
                   const int64_t* conn) {
+  return fecx_set_mesh_blocks(ctx, elem_kind, nd, nn, coords, 1, &ne, conn);
+}
+
+int fecx_set_mesh_blocks(fecx_ctx* ctx, int elem_kind, int nd, int64_t nn, const double* coords, int nblocks,
+                         const int64_t* block_ne, const int64_t* conn) {
   return guard(ctx, [&] {
     fecx::ElemInfo ei = fecx::elem_info(elem_kind);
     require(ei.kind >= 0, FECX_ERR_UNSUPPORTED, "unsupported element kind");
     require(nd == ei.nd, FECX_ERR_ARG, "coordinate dimension does not match the element kind");
+    require(nblocks >= 1 && nblocks <= FECX_MAX_BLOCKS && block_ne, FECX_ERR_ARG, "nblocks must be in 1..16");
+    std::vector<int64_t> bs(nblocks + 1, 0);
+    for (int b = 0; b < nblocks; ++b) {
+      require(block_ne[b] >= 0, FECX_ERR_ARG, "negative block size");
+      bs[b + 1] = bs[b] + block_ne[b];
+    }
+    const int64_t ne = bs[nblocks];
     require(nn > 0 && ne > 0 && coords && conn, FECX_ERR_ARG, "empty mesh");
     require(nn < (int64_t(1) << 31) && ne * ei.nnpe < (int64_t(1) << 31), FECX_ERR_ARG, "mesh exceeds int32 indexing");
     ctx->kind = elem_kind; ctx->nd = nd; ctx->nnpe = ei.nnpe; ctx->nq = ei.nq; ctx->nn = nn; ctx->ne = ne;
@@ -278,8 +300,11 @@ int fecx_set_mesh(fecx_ctx* ctx, int elem_kind, int nd, int64_t nn, const double
     }
     upload(ctx, "xhi", hi);
     upload(ctx, "xlo", lo);
+    ctx->block_start = bs;
+    ctx->block_mp.assign(nblocks, fecx::MatParams{});
+    ctx->block_set.assign(nblocks, 0);
     ctx->has_mesh = true;
-    ctx->has_dofs = ctx->has_pattern = false;
+    ctx->has_dofs = ctx->has_pattern = ctx->has_material = false;
   });
 }
 
@@ -301,11 +326,12 @@ int fecx_set_dofs(fecx_ctx* ctx, int nf, const int64_t* dirichlet, int64_t ndir)
     for (int64_t d = 1; d <= N; ++d)
       if (!isbc[d]) ctx->unknown_dofs.push_back(d);
     ctx->node_free.assign(ctx->nn, 1);
-    for (int64_t n = 0; n < ctx->nn; ++n) {
+    // condensed mode keeps every dof in the pattern, so any Dirichlet set is accepted there
+    for (int64_t n = 0; n < ctx->nn && !ctx->condensed; ++n) {
       int c = 0;
       for (int i = 0; i < nf; ++i) c += isbc[n * nf + i + 1];
       require(c == 0 || c == nf, FECX_ERR_UNSUPPORTED,
-              "each node's dofs must be all free or all Dirichlet when nf > 1");
+              "each node's dofs must be all free or all Dirichlet when nf > 1 (unless in condensed mode)");
       ctx->node_free[n] = c == 0;
     }
     std::vector<int64_t> u0(ctx->unknown_dofs.size()), d0(ndir);
@@ -315,6 +341,14 @@ int fecx_set_dofs(fecx_ctx* ctx, int nf, const int64_t* dirichlet, int64_t ndir)
     upload(ctx, "dirichlet_idx", d0);
     ctx->has_dofs = true;
     ctx->has_pattern = false;
+  });
+}
+
+int fecx_set_condensed(fecx_ctx* ctx, int on) {
+  return guard(ctx, [&] {
+    require(ctx->has_mesh, FECX_ERR_STATE, "fecx_set_mesh has not been called");
+    ctx->condensed = on != 0;
+    ctx->has_dofs = ctx->has_pattern = false;  // the DOF layout and the pattern depend on the mode
   });
 }
 
@@ -333,7 +367,7 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
     require((ctx->ne * ctx->pairs.npairs * ctx->nf * ctx->nf) < (int64_t(1) << 31), FECX_ERR_ARG,
             "element storage exceeds 2^31 words");
     fecx::build_host_pattern(ctx->hp, ctx->nn, ctx->ne, ctx->nnpe, ctx->conn0.data(), ctx->nf, ctx->node_free,
-                             ctx->pairs, ctx->nd, ctx->coords.data());
+                             ctx->pairs, ctx->nd, ctx->coords.data(), ctx->block_start);
     const int nf = ctx->nf;
     const auto& hp = ctx->hp;
     // column slots are bytes with 0xFF for "not a free node"; the reduction takes rows of up to 256 blocks
@@ -462,11 +496,16 @@ int fecx_get_coo_map(fecx_ctx* ctx, int64_t* map) {
   });
 }
 
-int fecx_set_material(fecx_ctx* ctx, int kind, const double* props, int nprops) {
-  return guard(ctx, [&] {
+// Physics and properties of block `block` (every block for block = -1); (re)allocates the quadrature state, set to
+// zero, when any block carries state.
+static void set_material(fecx_ctx* ctx, int block, int kind, const double* props, int nprops) {
     require(ctx->has_mesh, FECX_ERR_STATE, "fecx_set_mesh has not been called");
+    const int nb = (int)ctx->block_mp.size();
+    require(block >= -1 && block < nb, FECX_ERR_ARG, "block index out of range");
     require(nprops >= 0 && nprops <= 8 && (nprops == 0 || props), FECX_ERR_ARG, "bad properties");
     require(kind >= FECX_POISSON && kind <= FECX_NH_SURROGATE, FECX_ERR_UNSUPPORTED, "unsupported physics kind");
+    require(nb == 1 || (kind != FECX_J2_SURROGATE && kind != FECX_NH_SURROGATE), FECX_ERR_UNSUPPORTED,
+            "surrogate materials need a one-block mesh");
     fecx::MatParams mp{};
     mp.phys = kind;
     for (int i = 0; i < nprops; ++i) mp.p[i] = (float)props[i];
@@ -479,14 +518,41 @@ int fecx_set_material(fecx_ctx* ctx, int kind, const double* props, int nprops) 
     const bool j2 = kind == FECX_J2 || kind == FECX_J2_SURROGATE;
     if (nh) require(nprops >= 2, FECX_ERR_ARG, "neo-Hookean needs {lambda, mu}");
     if (j2) require(nprops >= 4, FECX_ERR_ARG, "J2 needs {E, nu, sigma_y, H}");
-    ctx->mp = mp;
-    ctx->ns = j2 ? 7 : 0;
+    for (int b = 0; b < nb; ++b)
+      if (block < 0 || b == block) { ctx->block_mp[b] = mp; ctx->block_set[b] = 1; }
+    ctx->mp = ctx->block_mp[0];
+    int ns = 0;
+    bool all = true;
+    for (int b = 0; b < nb; ++b) {
+      const int ph = ctx->block_mp[b].phys;
+      if (ctx->block_set[b] && (ph == FECX_J2 || ph == FECX_J2_SURROGATE)) ns = 7;
+      all = all && ctx->block_set[b];
+    }
+    ctx->ns = ns;
     if (ctx->ns && ctx->device >= 0) {
       const size_t bytes = sizeof(float) * ctx->ns * ctx->nq * ctx->ne;
       HIPCHK(hipMemset(dev(ctx, "state_old", bytes), 0, bytes));
       HIPCHK(hipMemset(dev(ctx, "state_new", bytes), 0, bytes));
     }
-    ctx->has_material = true;
+    ctx->has_material = all;
+}
+
+int fecx_set_material(fecx_ctx* ctx, int kind, const double* props, int nprops) {
+  return guard(ctx, [&] { set_material(ctx, -1, kind, props, nprops); });
+}
+
+int fecx_set_block_material(fecx_ctx* ctx, int block, int kind, const double* props, int nprops) {
+  return guard(ctx, [&] {
+    require(block >= 0, FECX_ERR_ARG, "block index out of range");
+    set_material(ctx, block, kind, props, nprops);
+  });
+}
+
+int fecx_get_num_blocks(fecx_ctx* ctx, int* nblocks) {
+  return guard(ctx, [&] {
+    require(ctx->has_mesh, FECX_ERR_STATE, "fecx_set_mesh has not been called");
+    require(nblocks != nullptr, FECX_ERR_ARG, "nblocks is NULL");
+    *nblocks = (int)ctx->block_mp.size();
   });
 }
 
@@ -596,8 +662,9 @@ int fecx_update_field_unknowns(fecx_ctx* ctx, const double* Uu, double* U) {
   return guard(ctx, [&] {
     check_device(ctx);
     require(ctx->has_dofs, FECX_ERR_STATE, "fecx_set_dofs has not been called");
-    fecx::launch_scatter_unknowns((int64_t)ctx->unknown_dofs.size(), get<int64_t>(ctx, "unknown_idx"), Uu, U,
-                                  ctx->stream);
+    const int64_t n = (int64_t)ctx->unknown_dofs.size();
+    if (ctx->condensed) fecx::launch_copy_indexed(n, get<int64_t>(ctx, "unknown_idx"), Uu, U, ctx->stream);
+    else fecx::launch_scatter_unknowns(n, get<int64_t>(ctx, "unknown_idx"), Uu, U, ctx->stream);
     HIPCHK(hipGetLastError());
   });
 }

@@ -75,14 +75,18 @@ struct HostPattern {
   std::vector<int32_t> eperm;       // storage position -> element
   std::vector<int32_t> epos;        // element -> storage position
   std::vector<int64_t> ekey;        // nn + 1: storage positions of the elements whose smallest node rank is < p
+                                    // (one element block only; empty otherwise)
+  std::vector<int64_t> block_pos;   // nblocks + 1: storage position range of each element block
   std::vector<int32_t> voff;        // nn + 1 : node -> incidences
   std::vector<int32_t> vent;        // storage position * nnpe + a, ascending element id per node
   std::vector<uint8_t> vslot;       // incidences x vstride bytes: column slot per local node b
 };
 
-// conn0: 0-based node ids, ne x nnpe row-major (element-major); node_free: nn flags; coords: nd x nn (FP64).
+// conn0: 0-based node ids, ne x nnpe row-major (element-major); node_free: nn flags; coords: nd x nn (FP64);
+// block_start: nblocks + 1 element-id offsets of the element blocks (blocks numbered one after the other).
 void build_host_pattern(HostPattern& hp, int64_t nn, int64_t ne, int nnpe, const int32_t* conn0, int nf,
-                        const std::vector<char>& node_free, const PairTable& pt, int nd, const double* coords);
+                        const std::vector<char>& node_free, const PairTable& pt, int nd, const double* coords,
+                        const std::vector<int64_t>& block_start);
 
 // ------------------------------------------------------------------ kernels
 // Work-groups are dealt to the 8 XCDs round-robin; remap block index wg so that each XCD walks a contiguous range of
@@ -148,6 +152,8 @@ int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s);
 // DOF plumbing
 void launch_scatter_unknowns(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s);
 void launch_gather_unknowns(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s);
+// dst[idx[i]] = src[idx[i]] (condensed mode: full-length unknown vectors)
+void launch_copy_indexed(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s);
 
 }  // namespace fecx
 
@@ -170,10 +176,12 @@ struct fecx_ctx {
   bool has_mesh = false;
   int kind = -1, nd = 0, nnpe = 0, nq = 0;
   int64_t nn = 0, ne = 0;
+  std::vector<int64_t> block_start;  // nblocks + 1 element-id offsets (one block: {0, ne})
   std::vector<int32_t> conn0;  // ne x nnpe, 0-based
   std::vector<double> coords;  // nd x nn
   // dofs
   bool has_dofs = false;
+  bool condensed = false;             // use_condensed: every dof stays in the pattern (fecx_set_condensed)
   int nf = 0;
   std::vector<int64_t> dirichlet;     // 1-based
   std::vector<int64_t> unknown_dofs;  // 1-based
@@ -183,9 +191,11 @@ struct fecx_ctx {
   fecx::PairTable pairs;
   fecx::HostPattern hp;
   int64_t nnz = 0;
-  // material
+  // material: one per element block (foreach_block's per-block physics and properties); mp is block 0's
   bool has_material = false;
   fecx::MatParams mp{};
+  std::vector<fecx::MatParams> block_mp;
+  std::vector<char> block_set;
   int ns = 0;
   // device buffers
   std::map<std::string, DevBuf> buf;

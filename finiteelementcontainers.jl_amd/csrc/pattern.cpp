@@ -78,7 +78,8 @@ static uint64_t spread3(uint64_t x) {  // 21 bits -> every third bit
 }
 
 void build_host_pattern(HostPattern& hp, int64_t nn, int64_t ne, int nnpe, const int32_t* conn0, int nf,
-                        const std::vector<char>& node_free, const PairTable& pt, int nd, const double* coords) {
+                        const std::vector<char>& node_free, const PairTable& pt, int nd, const double* coords,
+                        const std::vector<int64_t>& block_start) {
   (void)pt;
   hp = HostPattern();
   hp.nf = nf; hp.nnpe = nnpe; hp.nn = nn; hp.ne = ne;
@@ -116,17 +117,27 @@ void build_host_pattern(HostPattern& hp, int64_t nn, int64_t ne, int nnpe, const
       key[el] = m;
     }
   });
+  // Several element blocks (elements numbered block after block, as the blocks' connectivities are concatenated):
+  // each block takes a contiguous range of storage positions, in block order, sorted by that key within the block,
+  // so that one element-kernel launch per block runs that block's material over its range.
   {
-    std::vector<int64_t> cnt(nn + 1, 0);
-    for (int64_t el = 0; el < ne; ++el) cnt[key[el] + 1]++;
-    for (int64_t n = 0; n < nn; ++n) cnt[n + 1] += cnt[n];
-    hp.ekey = cnt;
+    const int nb = (int)block_start.size() - 1;
     hp.eperm.resize(ne);
     hp.epos.resize(ne);
-    for (int64_t el = 0; el < ne; ++el) {
-      const int64_t s = cnt[key[el]]++;
-      hp.eperm[s] = (int32_t)el;
-      hp.epos[el] = (int32_t)s;
+    hp.block_pos.assign(block_start.begin(), block_start.end());
+    std::vector<int64_t> cnt(nn + 1);
+    for (int b = 0; b < nb; ++b) {
+      const int64_t e0 = block_start[b], e1 = block_start[b + 1];
+      std::fill(cnt.begin(), cnt.end(), 0);
+      cnt[0] = e0;
+      for (int64_t el = e0; el < e1; ++el) cnt[key[el] + 1]++;
+      for (int64_t n = 0; n < nn; ++n) cnt[n + 1] += cnt[n];
+      if (nb == 1) hp.ekey = cnt;  // piece boundaries of pipelined assembly (one block only)
+      for (int64_t el = e0; el < e1; ++el) {
+        const int64_t s = cnt[key[el]]++;
+        hp.eperm[s] = (int32_t)el;
+        hp.epos[el] = (int32_t)s;
+      }
     }
   }
   // node -> incidences, ascending element id (counting sort over elements in order)

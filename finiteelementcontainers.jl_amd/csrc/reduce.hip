@@ -688,6 +688,14 @@ __global__ void gather_idx_kernel(int64_t n, const int64_t* idx, const double* s
   if (i < n) dst[i] = src[idx[i]];
 }
 
+__global__ void copy_idx_kernel(int64_t n, const int64_t* idx, const double* src, double* dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[idx[i]] = src[idx[i]];
+}
+void launch_copy_indexed(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s) {
+  if (n > 0) copy_idx_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, idx, src, dst);
+}
+
 void launch_scatter_unknowns(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s) {
   if (n > 0) scatter_idx_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, idx, src, dst);
 }

@@ -121,6 +121,43 @@ class TetMesh:
     element_conns = StructuredMesh.element_conns
 
 
+class BlockMesh:
+    """A mesh of several element blocks of one element type (the reference's FunctionSpace blocks, at most 16,
+    src/FunctionSpaces.jl:63): ``blocks`` is an ordered list of (name, conn NE_b x NNPE, 1-based); elements are
+    numbered block after block, so ``conn`` is the concatenation (Connectivity.data with per-block offsets,
+    src/Fields.jl:129-188)."""
+
+    def __init__(self, coords, blocks, element_type, nodeset_nodes=None, sideset_nodes=None):
+        if not 1 <= len(blocks) <= 16:
+            raise ValueError("a mesh holds 1..16 element blocks")
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64)
+        self.element_type = element_type
+        self.num_dims = self.coords.shape[1]
+        self.block_names = [b[0] for b in blocks]
+        self.block_conns = [np.ascontiguousarray(b[1], dtype=np.int64) for b in blocks]
+        self.conn = np.concatenate(self.block_conns, axis=0)
+        self.nodeset_nodes = dict(nodeset_nodes or {})
+        self.sideset_nodes = dict(sideset_nodes or {})
+
+    @property
+    def block_sizes(self):
+        return [c.shape[0] for c in self.block_conns]
+
+    num_nodes = StructuredMesh.num_nodes
+    num_elements = StructuredMesh.num_elements
+
+
+def split_into_blocks(mesh, labels, names=None):
+    """The elements of `mesh` split into blocks by an integer label per element (blocks in ascending label order,
+    element order kept within each block)."""
+    labels = np.asarray(labels)
+    keys = np.unique(labels)
+    names = names or [f"block_{k}" for k in keys]
+    blocks = [(nm, mesh.conn[labels == k]) for nm, k in zip(names, keys)]
+    return BlockMesh(mesh.coords, blocks, mesh.element_type, getattr(mesh, "nodeset_nodes", {}),
+                     getattr(mesh, "sideset_nodes", {}))
+
+
 _PERMS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
 _PERM_SIGN = [1, -1, -1, 1, 1, -1]
 

@@ -90,11 +90,33 @@ int fecx_set_stream(fecx_ctx* ctx, void* hip_stream);
 int fecx_set_mesh(fecx_ctx* ctx, int elem_kind, int nd, int64_t nn, const double* coords, int64_t ne,
                   const int64_t* conn);
 
+/* Up to FECX_MAX_BLOCKS element blocks of one element kind (the reference's FunctionSpace blocks, at most
+ * MAX_BLOCKS = 16, src/FunctionSpaces.jl:63): conn is the concatenation of the blocks' nnpe x block_ne[b]
+ * connectivities in block order (Connectivity.data with its per-block offsets, src/Fields.jl:129-188), so element
+ * ids run block after block as in the reference's pattern (SparsityPatterns.jl:53-117). Each block may carry its own
+ * physics (fecx_set_block_material), run block by block like foreach_block (src/Utils.jl:171-263).
+ * fecx_set_mesh is this call with one block. */
+#define FECX_MAX_BLOCKS 16
+int fecx_set_mesh_blocks(fecx_ctx* ctx, int elem_kind, int nd, int64_t nn, const double* coords, int nblocks,
+                         const int64_t* block_ne, const int64_t* conn);
+
+/* Number of element blocks of the mesh. */
+int fecx_get_num_blocks(fecx_ctx* ctx, int* nblocks);
+
 /* nf fields per node and the Dirichlet DOF list, sorted and unique, 1-based (the reference's
  * dirichlet_dofs(bcs), src/bcs/DirichletBCs.jl:396-398). Performs update_dofs!
- * (src/DofManagers.jl:227-298) without periodic pairs. For nf > 1 each node's dofs must be
- * all free or all constrained (FECX_ERR_UNSUPPORTED otherwise). */
+ * (src/DofManagers.jl:227-298) without periodic pairs. Outside condensed mode, for nf > 1 each node's
+ * dofs must be all free or all constrained (FECX_ERR_UNSUPPORTED otherwise). */
 int fecx_set_dofs(fecx_ctx* ctx, int nf, const int64_t* dirichlet_dofs, int64_t n_dirichlet);
+
+/* Condensed mode, SparseMatrixAssembler(var; use_condensed = true) (src/assemblers/SparseMatrixAssembler.jl:126-133,
+ * DofManager{true}, src/DofManagers.jl:41-42). on = 1 keeps every DOF in the pattern: fecx_build_pattern then makes
+ * the N x N CSR over all dofs, Dirichlet rows and columns included, as a condensed assembler never reduces its
+ * pattern (SparseMatrixAssembler.jl:246-251); the Dirichlet set may then fix single components of a node. The
+ * unknown vectors of fecx_update_field_unknowns are full length (U[unknown_dofs] = Uu[unknown_dofs], the condensed
+ * kernel of update_field_unknowns!, src/DofManagers.jl:328-364), and the accessors adjust the assembled results with
+ * the fecx_adjust_*_for_constraints calls below. Takes effect at the next fecx_set_dofs (called again after this). */
+int fecx_set_condensed(fecx_ctx* ctx, int on);
 
 /* n_unknowns and, if unknown_dofs != NULL, the unknown DOF ids (1-based, ascending):
  * DofManager.unknown_dofs after update_dofs! (src/DofManagers.jl:262-263). */
@@ -107,9 +129,12 @@ int fecx_get_unknown_dofs(fecx_ctx* ctx, int64_t* n_unknowns, int64_t* unknown_d
  * included): FECX_ERR_UNSUPPORTED otherwise. */
 int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz);
 
-/* CSR pattern (n_unknowns+1 row pointers, nnz column indices) with the given index base
+/* CSR pattern (n_unknowns+1 row pointers, N+1 in condensed mode; nnz column indices) with the given index base
  * (1 = SparseMatrixCSR(csc) convention, 0 = C). Bit-exact with SparseArrays.sparse! followed by
- * SparseMatrixCSR (src/assemblers/SparsityPatterns.jl:301-308,326-329). */
+ * SparseMatrixCSR (src/assemblers/SparsityPatterns.jl:301-308,326-329). The element pattern is structurally
+ * symmetric, so the same arrays are the CSC colptr/rowval of sparse! itself (sparse_matrix_type = :csc, the
+ * reference's default, SparseMatrixAssembler.jl:66, _csc_matrix SparsityPatterns.jl:301-308); the tangents and masses
+ * assembled here are symmetric bit for bit, so the CSR values are also the CSC nzval. */
 int fecx_get_pattern(fecx_ctx* ctx, int64_t* rowptr, int64_t* colval, int index_base);
 
 /* The COO->CSR index map: for every element-storage slot (ne * (nf*nnpe)^2 entries, element-major,
@@ -118,9 +143,15 @@ int fecx_get_pattern(fecx_ctx* ctx, int64_t* rowptr, int64_t* colval, int index_
  * (dropped by _update_dofs!, SparsityPatterns.jl:195-213). */
 int fecx_get_coo_map(fecx_ctx* ctx, int64_t* map);
 
-/* Physics kind and properties for the block; also (re)allocates the quadrature state
+/* Physics kind and properties for every block; also (re)allocates the quadrature state
  * (NS x NQ x NE, FP32) set to zero when the kind carries state (src/Parameters.jl:1-23). */
 int fecx_set_material(fecx_ctx* ctx, int kind, const double* props, int nprops);
+
+/* Physics kind and properties of element block `block` (0-based): the per-block entries of the Parameters'
+ * physics and properties tuples (src/Parameters.jl:37-73) that foreach_block hands to each block
+ * (src/Utils.jl:171-189). Assembly needs every block set. The state (NS x NQ x NE over all blocks, element-id
+ * order) has NS = 7 when any block is J2, and is set to zero. Surrogate kinds need a one-block mesh. */
+int fecx_set_block_material(fecx_ctx* ctx, int block, int kind, const double* props, int nprops);
 
 /* Number of state variables per quadrature point and quadrature points per element. */
 int fecx_get_state_size(fecx_ctx* ctx, int* ns, int* nq);
@@ -179,7 +210,18 @@ int fecx_assemble_lumped_mass(fecx_ctx* ctx, double density, double* M);
 int fecx_assemble_mass_diagonal(fecx_ctx* ctx, double density, double* D);
 int fecx_assemble_mass_action(fecx_ctx* ctx, double density, const double* V, double* MV);
 
-/* U[unknown_dofs] = Uu: update_field_unknowns! (src/DofManagers.jl:328-411). */
+/* Condensed-mode adjustments with constraint_storage c (1 at the Dirichlet dofs; src/assemblers/Utils.jl:53-167), on
+ * device arrays, enqueued and synchronised. stiffness(asm) / mass(asm): every line of a constrained dof of the N x N
+ * matrix (a CSR row, or a CSC column: the same values, see fecx_get_pattern) is multiplied by 1 - c = 0 and its
+ * diagonal entry gets + penalty, penalty = penalty_scale tr(A) / N (the reference uses 1e6) with the trace taken
+ * before the adjustment (Utils.jl:88-145). residual(asm): b = (1 - c) b (Utils.jl:161-167). hvp(asm, v):
+ * Av = (1 - c) Av + c v (Utils.jl:150-159). FECX_ERR_STATE unless the context is in condensed mode. */
+int fecx_adjust_matrix_for_constraints(fecx_ctx* ctx, double* vals, double penalty_scale);
+int fecx_adjust_vector_for_constraints(fecx_ctx* ctx, double* b);
+int fecx_adjust_action_for_constraints(fecx_ctx* ctx, double* Av, const double* v);
+
+/* U[unknown_dofs] = Uu: update_field_unknowns! (src/DofManagers.jl:328-411); in condensed mode Uu is full length and
+ * U[unknown_dofs] = Uu[unknown_dofs]. */
 int fecx_update_field_unknowns(fecx_ctx* ctx, const double* Uu, double* U);
 
 /* U[dirichlet_dofs] = vals (n_dirichlet values, in the order given to fecx_set_dofs):

@@ -622,6 +622,159 @@ i64 orc_sparse_csr(i64 nrows, i64 ncoo, const i64* Is, const i64* Js, const doub
   return nnz;
 }
 
+// sparse!(I, J, V, n, n, +) alone: the CSC matrix of the default sparse_matrix_type = :csc (_csc_matrix,
+// SparsityPatterns.jl:301-308), pass 1 and pass 2 of orc_sparse_csr without the final CSC -> CSR step. Outputs colptr
+// (n+1, 1-based), rowval (nnz, 1-based, ascending within each column) and nzval. Returns nnz; call with colptr NULL to
+// only count.
+i64 orc_sparse_csc(i64 nrows, i64 ncoo, const i64* Is, const i64* Js, const double* V, i64* colptr, i64* rowval,
+                   double* nzval) {
+  const i64 m = nrows;
+  std::vector<i64> crp(m + 2, 0);
+  for (i64 k = 0; k < ncoo; ++k) crp[Is[k] + 1]++;
+  crp[1] = 1;
+  for (i64 i = 1; i <= m; ++i) crp[i + 1] += crp[i];
+  std::vector<i64> ccol(ncoo);
+  std::vector<double> cval(ncoo);
+  std::vector<i64> fill(crp.begin(), crp.end());
+  for (i64 k = 0; k < ncoo; ++k) {
+    const i64 pos = fill[Is[k]]++;
+    ccol[pos - 1] = Js[k];
+    cval[pos - 1] = V ? V[k] : 0.0;
+  }
+  std::vector<i64> klast(m + 1, 0), rstart(m + 2);
+  i64 wr = 0;
+  for (i64 i = 1; i <= m; ++i) {
+    rstart[i] = wr + 1;
+    for (i64 p = crp[i]; p < crp[i + 1]; ++p) {
+      const i64 j = ccol[p - 1];
+      if (klast[j] < rstart[i]) {
+        ++wr;
+        klast[j] = wr;
+        ccol[wr - 1] = j;
+        cval[wr - 1] = cval[p - 1];
+      } else {
+        cval[klast[j] - 1] += cval[p - 1];
+      }
+    }
+  }
+  rstart[m + 1] = wr + 1;
+  const i64 nnz = wr;
+  if (!colptr) return nnz;
+  std::vector<i64> ccp(m + 2, 0);
+  for (i64 k = 0; k < nnz; ++k) ccp[ccol[k] + 1]++;
+  ccp[1] = 1;
+  for (i64 j = 1; j <= m; ++j) ccp[j + 1] += ccp[j];
+  std::vector<i64> f2(ccp.begin(), ccp.end());
+  for (i64 i = 1; i <= m; ++i)
+    for (i64 p = rstart[i]; p < rstart[i + 1]; ++p) {
+      const i64 pos = f2[ccol[p - 1]]++;
+      rowval[pos - 1] = i;
+      nzval[pos - 1] = cval[p - 1];
+    }
+  for (i64 j = 0; j <= m; ++j) colptr[j] = ccp[j + 1];
+  return nnz;
+}
+
+// Condensed-mode constraint adjustments (use_condensed = true; src/assemblers/Utils.jl:53-167). c is the
+// constraint_storage vector: 1 at Dirichlet dofs, 0 elsewhere (SparseMatrixAssembler.jl:93-94,246-248).
+// Matrix, compressed form (ptr, idx, nz) of n x n: for every compressed line k (a CSR row, or a CSC column),
+// nz = (1 - c[k]) nz, and its diagonal entry gets + penalty c[k], penalty = penalty_scale tr(A) / n with the trace
+// taken before the adjustment (Utils.jl:53-145; _sptrace = tr, :43-51). The CSR form zeroes rows, the CSC form
+// columns, exactly as the reference's methods for the two types do.
+void orc_adjust_matrix(i64 n, const i64* ptr, const i64* idx, double* nz, const double* c, double penalty_scale) {
+  double tr = 0.0;
+  for (i64 k = 0; k < n; ++k)
+    for (i64 p = ptr[k]; p < ptr[k + 1]; ++p)
+      if (idx[p - 1] == k + 1) tr += nz[p - 1];
+  const double penalty = penalty_scale * tr / (double)n;
+  for (i64 k = 0; k < n; ++k)
+    for (i64 p = ptr[k]; p < ptr[k + 1]; ++p) {
+      nz[p - 1] = (1.0 - c[k]) * nz[p - 1];
+      if (idx[p - 1] == k + 1) nz[p - 1] = nz[p - 1] + penalty * c[k];
+    }
+}
+
+// b = (1 - c) b (Utils.jl:161-167); Av = (1 - c) Av + c v (Utils.jl:150-159).
+void orc_adjust_vector(i64 n, double* b, const double* c) {
+  for (i64 k = 0; k < n; ++k) b[k] = (1.0 - c[k]) * b[k];
+}
+void orc_adjust_action(i64 n, double* Av, const double* c, const double* v) {
+  for (i64 k = 0; k < n; ++k) Av[k] = (1.0 - c[k]) * Av[k] + c[k] * v[k];
+}
+
+// The same CSR as pattern -> gather of the element storage -> orc_sparse_csr, formed row by row so that it runs in
+// parallel and needs no COO arrays (full-size checks: 5.1e8 COO entries at hex8 96^3). Row r of unknown dof
+// (node n, component d) sums, over n's incident elements in ascending element id and then the element's columns j,
+// storage[e NDOF^2 + i NDOF + j] with i = a nf + d (SparsityPatterns.jl:72-84 enumeration): the order in which
+// sparse! combines duplicates (its counting sort by row keeps COO order, SparsityPatterns.jl:301-308), so the values
+// are bit-identical to orc_sparse_csr's. Columns ascend within each row (SparseMatrixCSR, :326-329).
+// Call with colval == NULL to receive rowptr (n_unknowns + 1, 1-based) and return nnz; then again with all arrays.
+i64 orc_sparse_csr_rows(int nf, int nnpe, i64 nn, i64 ne, const i64* conn, const i64* dof_to_unknown,
+                        i64 n_unknowns, const double* storage, i64* rowptr, i64* colval, double* vals) {
+  const int NDOF = nf * nnpe;
+  std::vector<i64> noff(nn + 1, 0), nel(ne * nnpe);
+  for (i64 k = 0; k < ne * nnpe; ++k) noff[conn[k]]++;
+  for (i64 i = 0; i < nn; ++i) noff[i + 1] += noff[i];
+  {
+    std::vector<i64> fill(noff.begin(), noff.end() - 1);
+    for (i64 e = 0; e < ne; ++e)
+      for (int a = 0; a < nnpe; ++a) nel[fill[conn[e * nnpe + a] - 1]++] = e * nnpe + a;  // ascending e per node
+  }
+  const bool count = colval == nullptr;
+  if (count) {
+    for (i64 r = 0; r <= n_unknowns; ++r) rowptr[r] = 0;
+  }
+#pragma omp parallel
+  {
+    std::vector<i64> cols;
+    std::vector<i64> mark(nn * nf, -1);  // unknown column -> position in the row
+    std::vector<double> acc;
+#pragma omp for schedule(dynamic, 256)
+    for (i64 n = 0; n < nn; ++n) {
+      for (int d = 0; d < nf; ++d) {
+        const i64 r = dof_to_unknown[nf * n + d];
+        if (r <= 0) continue;
+        cols.clear();
+        for (i64 t = noff[n]; t < noff[n + 1]; ++t) {
+          const i64 e = nel[t] / nnpe;
+          for (int b = 0; b < nnpe; ++b)
+            for (int k = 0; k < nf; ++k) {
+              const i64 c = dof_to_unknown[nf * (conn[e * nnpe + b] - 1) + k];
+              if (c > 0 && mark[c - 1] < 0) { mark[c - 1] = 0; cols.push_back(c); }
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        if (count) {
+          rowptr[r] = (i64)cols.size();
+        } else {
+          const i64 base = rowptr[r - 1] - 1;
+          acc.assign(cols.size(), 0.0);
+          for (size_t q = 0; q < cols.size(); ++q) mark[cols[q] - 1] = (i64)q;
+          for (i64 t = noff[n]; t < noff[n + 1]; ++t) {
+            const i64 e = nel[t] / nnpe;
+            const int i = (int)(nel[t] % nnpe) * nf + d;
+            const double* s = storage + e * NDOF * NDOF + (i64)i * NDOF;
+            for (int j = 0; j < NDOF; ++j) {
+              const i64 c = dof_to_unknown[nf * (conn[e * nnpe + j / nf] - 1) + j % nf];
+              if (c > 0) acc[mark[c - 1]] += s[j];
+            }
+          }
+          for (size_t q = 0; q < cols.size(); ++q) {
+            colval[base + (i64)q] = cols[q];
+            vals[base + (i64)q] = acc[q];
+          }
+        }
+        for (i64 c : cols) mark[c - 1] = -1;
+      }
+    }
+  }
+  if (count) {
+    rowptr[0] = 1;
+    for (i64 r = 1; r <= n_unknowns; ++r) rowptr[r] += rowptr[r - 1];
+  }
+  return rowptr[n_unknowns] - 1;
+}
+
 // ------------------------------------------------------------------ element loops
 // Residual into a full NF x NN field (Assemblers.jl:402-432 + :72-87, atomic adds when threaded,
 // src/Utils.jl:1-21). U: full field.

@@ -31,6 +31,8 @@ def lib():
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.orc_pattern_count.restype = ctypes.c_int64
         _lib.orc_sparse_csr.restype = ctypes.c_int64
+        _lib.orc_sparse_csr_rows.restype = ctypes.c_int64
+        _lib.orc_sparse_csc.restype = ctypes.c_int64
         _lib.orc_update_dofs.restype = ctypes.c_int64
         _lib.orc_cg.restype = ctypes.c_int
     return _lib
@@ -103,6 +105,68 @@ def sparse_csr(nrows, Is, Js, V=None, want_map=False):
     cmap = np.zeros(n, dtype=np.int64) if want_map else None
     lib().orc_sparse_csr(_i64(nrows), _i64(n), _p(Is), _p(Js), _p(Vv), _p(rowptr), _p(colval), _p(vals), _p(cmap))
     return rowptr, colval, vals, cmap
+
+
+def sparse_csc(nrows, Is, Js, V=None):
+    """sparse!(Is, Js, V, n, n, +): the CSC of sparse_matrix_type = :csc, (colptr, rowval) 1-based, nzval."""
+    Is = np.ascontiguousarray(Is, dtype=np.int64)
+    Js = np.ascontiguousarray(Js, dtype=np.int64)
+    Vv = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+    n = len(Is)
+    nnz = lib().orc_sparse_csc(_i64(nrows), _i64(n), _p(Is), _p(Js), None, None, None, None)
+    colptr = np.zeros(nrows + 1, dtype=np.int64)
+    rowval = np.zeros(nnz, dtype=np.int64)
+    nzval = np.zeros(nnz)
+    lib().orc_sparse_csc(_i64(nrows), _i64(n), _p(Is), _p(Js), _p(Vv), _p(colptr), _p(rowval), _p(nzval))
+    return colptr, rowval, nzval
+
+
+def constraint_storage(ndofs, dirichlet):
+    """The condensed assembler's constraint_storage: 1.0 at the Dirichlet dofs (1-based), 0.0 elsewhere."""
+    c = np.zeros(ndofs)
+    c[np.asarray(dirichlet, dtype=np.int64) - 1] = 1.0
+    return c
+
+
+def adjust_matrix(ptr, idx, nz, c, penalty_scale=1e6):
+    """Condensed-mode matrix adjustment in place on a compressed (CSR rows or CSC columns) 1-based matrix."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    assert nz.dtype == np.float64 and nz.flags.c_contiguous
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    lib().orc_adjust_matrix(_i64(len(ptr) - 1), _p(ptr), _p(idx), _p(nz), _p(c), ctypes.c_double(penalty_scale))
+    return nz
+
+
+def adjust_vector(b, c):
+    assert b.dtype == np.float64 and b.flags.c_contiguous
+    lib().orc_adjust_vector(_i64(len(b)), _p(b), _p(np.ascontiguousarray(c, dtype=np.float64)))
+    return b
+
+
+def adjust_action(Av, c, v):
+    assert Av.dtype == np.float64 and Av.flags.c_contiguous
+    lib().orc_adjust_action(_i64(len(Av)), _p(Av), _p(np.ascontiguousarray(c, dtype=np.float64)),
+                            _p(np.ascontiguousarray(v, dtype=np.float64)))
+    return Av
+
+
+def sparse_csr_rows(nf, conn, dof_to_unknown, n_unknowns, storage):
+    """The CSR of the element storage, as pattern -> gather_slots -> sparse_csr gives it (same pattern, same bits),
+    formed row by row in parallel without COO arrays: (rowptr, colval) 1-based, vals."""
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    d2u = np.ascontiguousarray(dof_to_unknown, dtype=np.int64)
+    storage = np.ascontiguousarray(storage, dtype=np.float64)
+    ne, nnpe = conn.shape
+    nn = len(d2u) // nf
+    rowptr = np.zeros(n_unknowns + 1, dtype=np.int64)
+    args = (ctypes.c_int(nf), ctypes.c_int(nnpe), _i64(nn), _i64(ne), _p(conn), _p(d2u), _i64(n_unknowns),
+            _p(storage), _p(rowptr))
+    nnz = lib().orc_sparse_csr_rows(*args, None, None)
+    colval = np.zeros(nnz, dtype=np.int64)
+    vals = np.zeros(nnz)
+    lib().orc_sparse_csr_rows(*args, _p(colval), _p(vals))
+    return rowptr, colval, vals
 
 
 def _common(elem_kind, X, conn, nf, phys, props):

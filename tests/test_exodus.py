@@ -125,10 +125,11 @@ def test_assembly_on_file_mesh(fec, orc, gpu, path, physics):
     fec.assemble_matrix_free_action_(asm, fec.stiffness_action, Uu, Vu, p)
     KV = fec.hvp(asm, Vu).cpu().numpy()
 
-    def rel(a, b):
-        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    def rel(a, b):  # the larger of the relative 2-norm and max-norm errors
+        return max(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)),
+                   float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)))
 
     assert np.array_equal(K.rowptr.cpu().numpy() + 1, rp) and np.array_equal(K.colval.cpu().numpy() + 1, cv)
     errs = (rel(R, R_ref), rel(K.values.cpu().numpy(), vals), rel(KV, KV_ref))
     print(f"[gpu exodus] R={errs[0]:.2e} K={errs[1]:.2e} KV={errs[2]:.2e}", flush=True)
-    assert errs[0] < 2e-5 and errs[1] < 2e-6 and errs[2] < 2e-6
+    assert max(errs) < 1e-6

@@ -1,21 +1,24 @@
 """GPU parity: residual, CSR Jacobian, assembled and matrix-free actions from libfecx.so (FP32 element kernels,
 FP64 reductions) against the FP64 oracle on the same seeded meshes and fields.
 
-Tolerances (relative, 2-norm over all entries): Jacobian values 2e-6, actions 2e-6, residual 2e-5 -- FP32 element
-arithmetic (unit roundoff 6e-8) summed over <= 8 elements x 8 quadrature points; the residual bound is looser because
-interior residuals of smooth fields are sums of cancelling element contributions. The sparsity pattern is compared
-bit-exactly."""
+Tolerance: 1e-6 relative error against FP64 for every output, in the 2-norm over all entries and in the max-norm
+(largest entry error over the largest entry) -- the bound the FP32 element kernels are held to. FP32 element
+arithmetic (unit roundoff 6e-8) summed over <= 24 elements x 8 quadrature points lands near 1e-7 at these sizes. The
+sparsity pattern is compared bit-exactly."""
 import numpy as np
 import pytest
 import torch
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"R": 2e-5, "K": 2e-6, "KV": 2e-6}
+TOL = {"R": 1e-6, "K": 1e-6, "KV": 1e-6}
 
 
 def rel(a, b):
-    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    """max of the relative 2-norm and max-norm errors"""
+    e2 = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    em = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) if len(b) else 0.0
+    return max(e2, em)
 
 
 def _kind(orc, mesh):
