@@ -66,11 +66,16 @@ def kernel_bytes(wl, asm, with_r=True, with_k=True):
     state = 2 * 4 * 7 * 8 * ne if wl.physics.kind == 3 else 0
     kel = 4 * npairs * nf * nf * ne if with_k else 0
     rel_ = 4 * ndof * ne if with_r else 0
-    return {
+    out = {
         "element": 4 * nnpe * ne + 2 * 4 * nd * nn + 8 * nf * nn + rel_ + kel + state,
         "reduce": 16 * nn + (4 + vstride) * ne * nnpe + kel + rel_
         + (8 * asm.nnz if with_k else 0) + (8 * nf * nn if with_r else 0),
     }
+    # the fused, atomic and coloured designs do the whole step: their algorithmic bytes are the step's compulsory bytes
+    name = getattr(wl, "name", None)
+    if with_r and with_k and name in STEP_BYTES:
+        out["fused"] = out["atomic"] = out["coloured"] = STEP_BYTES[name] * ne
+    return out
 
 
 def pmc_traffic(workload, kernel):
@@ -200,6 +205,24 @@ def operator_timings(fec, torch, name, wl, asm, p, Uu):
     return out
 
 
+def hbm_ceiling(fec, torch, asm, gib=4, reps=10):
+    """The device's HBM ceilings measured by the library's hand-written streaming kernels (fecx_measure_bandwidth) on a
+    `gib` GiB buffer: read, write, and copy (bytes read + bytes written), GB/s."""
+    import ctypes
+    nbytes = gib << 30
+    buf = torch.empty(nbytes // 8, dtype=torch.float64, device=asm.device)
+    buf.fill_(1.0)
+    out = {}
+    for op, name in enumerate(("read", "write", "copy")):
+        ms = ctypes.c_double(0.0)
+        asm.ctx.call("fecx_measure_bandwidth", ctypes.c_int(op), ctypes.c_void_p(buf.data_ptr()),
+                     ctypes.c_int64(nbytes), ctypes.c_int(reps), ctypes.byref(ms))
+        out[name] = nbytes / (ms.value * 1e-3) / 1e9
+    del buf
+    torch.cuda.empty_cache()
+    return out
+
+
 def gemm_peak_tflops(torch, n=8192, reps=10):
     """Measured BF16 dense GEMM throughput (torch.matmul on contiguous n x n operands): the attainable MFMA figure
     next to the 2.5 PFLOP/s spec."""
@@ -320,12 +343,16 @@ def main():
     ap.add_argument("--workload", default="neohookean_hex8", choices=sorted(SIZES))
     ap.add_argument("--n", type=int, default=None, help="elements per direction (default: the config's size)")
     ap.add_argument("--chunks", type=int, default=0, help="pipelined pieces per matrix assembly (0 or 1: one piece)")
+    ap.add_argument("--scatter", default="fused", choices=["fused", "two_phase", "atomic", "coloured"],
+                    help="scatter design of the matrix assembly (fused: structured hex8 meshes; two-phase elsewhere)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the full-size parity and CPU legs (profiling runs)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg (J2, tet10 NH)")
     ap.add_argument("--no-operators", action="store_true", help="skip the timings of the other entry points")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (HBM counter calibration)")
+    ap.add_argument("--hbm-ceiling", action="store_true",
+                    help="only measure the HBM read / write / copy ceilings with the library's streaming kernels")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -344,6 +371,11 @@ def main():
 
     import torch
     fec = ge.load_package()
+    if args.hbm_ceiling:
+        wl = fec.workloads.poisson_hex8(2)
+        asm, _ = build_assembler(fec, wl)
+        print(json.dumps({"hbm_ceiling_gbs": hbm_ceiling(fec, torch, asm), "buffer_gib": 4, "reps": 10}), flush=True)
+        return
     name = args.workload
     n_full, n_sample = SIZES[name]
     n = args.n or n_full
@@ -353,7 +385,13 @@ def main():
     asm, p = build_assembler(fec, wl)
     Uu = torch.from_numpy(wl.U[asm.dof.unknown_dofs - 1]).to(asm.device)
     fec.set_chunks(asm, args.chunks)
-    log(f"[bench] setup {time.time() - t0:.1f}s  NE={wl.num_elements} nnz={asm.nnz}")
+    import ctypes
+    design = ["fused", "two_phase", "atomic", "coloured"].index(args.scatter)
+    asm.ctx.call("fecx_set_scatter", ctypes.c_int(design))
+    path = ctypes.c_int(-1)
+    asm.ctx.call("fecx_get_scatter_path", ctypes.byref(path))
+    scatter_path = ["fused", "two_phase", "atomic", "coloured"][path.value]
+    log(f"[bench] setup {time.time() - t0:.1f}s  NE={wl.num_elements} nnz={asm.nnz}  scatter {scatter_path}")
 
     for _ in range(args.warmup):
         fec.assemble_vector_and_stiffness_(asm, Uu, p)
@@ -388,11 +426,12 @@ def main():
     # The roofline of the step's assembly kernels together (the DOF update excluded), priced on the compulsory bytes
     # per element of residual + tangent: int32 connectivity, FP32 coordinates, FP64 u, FP64 outputs written once, the
     # int32 COO->CSR map (NDOF^2 per element) and FP32 state.
-    kern_ms = sum(v["ms_per_step"] for k, v in kernels.items() if k in ("element", "reduce", "fused"))
+    STEP_KERNELS = ("element", "reduce", "fused", "atomic", "coloured")
+    kern_ms = sum(v["ms_per_step"] for k, v in kernels.items() if k in STEP_KERNELS)
     bpe = STEP_BYTES.get(name)
     traffic = None
-    if n == n_full and all("traffic" in v for k, v in kernels.items() if k in ("element", "reduce", "fused")):
-        traffic = sum(v["traffic"] for k, v in kernels.items() if k in ("element", "reduce", "fused"))
+    if n == n_full and all("traffic" in v for k, v in kernels.items() if k in STEP_KERNELS):
+        traffic = sum(v["traffic"] for k, v in kernels.items() if k in STEP_KERNELS)
     out = {
         "metric": METRIC,
         "value": wl.num_elements / dt,
@@ -408,14 +447,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{name} {n}^3" if wl.mesh.num_dims == 3 else f"{name} {n}^2",
                    "elements": int(wl.num_elements), "nnz": int(asm.nnz), "step": "residual + CSR Jacobian",
-                   "element_math": "fp32", "reductions": "fp64", "chunks": max(1, args.chunks)},
+                   "element_math": "fp32", "reductions": "fp64", "chunks": max(1, args.chunks),
+                   "scatter": scatter_path},
         "kernels": kernels,
     }
     if bpe:
         ach = bpe * wl.num_elements / (kern_ms * 1e-3) / 1e9
-        out["roofline"] = {"bound": "hbm", "kernel": "+".join(k for k in kernels if k in ("element", "reduce", "fused")),
+        out["roofline"] = {"bound": "hbm", "kernel": "+".join(k for k in kernels if k in STEP_KERNELS),
                            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                           "traffic": traffic, "bytes_per_element": bpe, "kernel_ms_per_step": kern_ms}
+                           "traffic": traffic, "bytes_per_element": bpe, "kernel_ms_per_step": kern_ms,
+                           "measured_ceiling_gbs": hbm_ceiling(fec, torch, asm, gib=2, reps=5)}
         eff = bpe * wl.num_elements / dt / 1e9
         out["step_roofline"] = {"bytes_per_element": bpe, "achieved": eff, "frac": eff / HBM_PEAK_GBS,
                                 "basis": "wall time per step (DOF update and host synchronisation included)"}

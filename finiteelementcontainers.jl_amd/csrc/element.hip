@@ -130,25 +130,25 @@ struct PairJobs {
   }
 };
 
-template <int KIND, int NF, int PHYS, int MODE>
-__global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
+// The element computation of one 256-thread work-group, G lanes per element (gather, quadrature points, node phase).
+// Lane group grp = threadIdx.x / G evaluates the element at storage position e (when `valid`). Every element row
+// (residual, action, diagonal) leaves through rel_out(word a * NF + i, value); the element blocks (DO_K) are left in
+// work.K, lane-major per element, when this returns (after a work-group barrier). Used by element_kernel (two-phase
+// assembly) and by fused_hex8_kernel (which reduces the blocks from LDS).
+template <int KIND, int NF, int PHYS, int MODE, class LDS, class WORK, class RelOut>
+__device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e, bool valid, LDS* lds, WORK& work,
+                                             RelOut rel_out, bool& bad, int tid) {
   using T = ET<KIND>;
   using M = Mat<PHYS>;
-  using LDS = ElemLDS<KIND, NF, PHYS, MODE>;
-  constexpr int ND = T::ND, NNPE = T::NNPE, NQ = T::NQ, G = T::G, EPW = ELEM_THREADS / G, NDOF = NNPE * NF;
+  constexpr int ND = T::ND, NNPE = T::NNPE, NQ = T::NQ, G = T::G;
   constexpr int D = NF * ND;
   constexpr bool DO_R = LDS::DO_R, DO_K = LDS::DO_K, DO_MF = LDS::DO_MF, DO_ASM = LDS::DO_ASM, DO_D = LDS::DO_D;
   constexpr bool DO_G = (MODE & MODE_QPGRAD) != 0;
   constexpr bool NEED_C = LDS::NEED_C, GS = LDS::GS, SC = LDS::SC, QR = LDS::QR, QM = LDS::QM, NOQP = LDS::NOQP;
-  __shared__ LDS lds[EPW];
-  __shared__ __attribute__((aligned(16))) ElemWork<LDS, EPW, NQ> work;
   constexpr int KW = LDS::KW;
 
-  const int lane = threadIdx.x % G, grp = threadIdx.x / G;
-  const int64_t e = args.ebeg + (int64_t)blockIdx.x * EPW + grp;  // element storage position
-  const bool valid = e < args.ne;
+  const int lane = tid % G, grp = tid / G;
   auto& L = lds[grp];
-  bool bad = false;
 
   // The node lanes' view of quadrature point Q: its tangent data C (the side scalars C[S0 .. S0+NS), or all of it in
   // the compact form), and node b's vector vb (staged, or formed here from g_b by the same prep as the staged form).
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
           }
         scatter_points8<NF>(t, q, ra);
 #pragma unroll
-        for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + q * NF + i] = ra[i];
+        for (int i = 0; i < NF; ++i) rel_out(q * NF + i, ra[i]);
       }
     }
     if constexpr (GS) {
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
           }
         scatter_points8<NF>(t, q, ra);
 #pragma unroll
-        for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + q * NF + i] = ra[i];
+        for (int i = 0; i < NF; ++i) rel_out(q * NF + i, ra[i]);
       } else {
         if (r == 0) {
 #pragma unroll
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         }
       }
 #pragma unroll
-      for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + a * NF + i] = out[i];
+      for (int i = 0; i < NF; ++i) rel_out(a * NF + i, out[i]);
     }
   }
   if constexpr (DO_K || DO_ASM) {
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
         M::template pair<ND>(sa, va, acc);
       }
 #pragma unroll
-      for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + a * NF + i] = acc[i * NF + i];
+      for (int i = 0; i < NF; ++i) rel_out(a * NF + i, acc[i * NF + i]);
     }
   }
   if constexpr (DO_ASM) {
@@ -587,15 +587,34 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
           for (int k = 0; k < NF; ++k) kv[i] = fmaf(pr.ab ? blk[i * NF + k] : blk[k * NF + i], L.v[b][k], kv[i]);
       }
 #pragma unroll
-      for (int i = 0; i < NF; ++i) args.Rel[e * NDOF + lane * NF + i] = kv[i];
+      for (int i = 0; i < NF; ++i) rel_out(lane * NF + i, kv[i]);
     }
   }
   if constexpr (DO_K) {
-    // stage the blocks in LDS (over the consumed quadrature data), then the whole wavefront copies its elements'
-    // contiguous storage range with consecutive lanes on consecutive words
+    // stage the blocks in LDS, over the consumed quadrature data
     __syncthreads();
     if (valid && lane < PJ::LANES) stage_blocks(work.K + grp * KW);
     __syncthreads();
+  }
+}
+
+template <int KIND, int NF, int PHYS, int MODE>
+__global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
+  using T = ET<KIND>;
+  using LDS = ElemLDS<KIND, NF, PHYS, MODE>;
+  constexpr int NNPE = T::NNPE, NQ = T::NQ, G = T::G, EPW = ELEM_THREADS / G, NDOF = NNPE * NF;
+  constexpr bool DO_K = LDS::DO_K;
+  constexpr int KW = LDS::KW;
+  __shared__ LDS lds[EPW];
+  __shared__ __attribute__((aligned(16))) ElemWork<LDS, EPW, NQ> work;
+  const int grp = threadIdx.x / G;
+  const int64_t e = args.ebeg + (int64_t)blockIdx.x * EPW + grp;  // element storage position
+  const bool valid = e < args.ne;
+  bool bad = false;
+  element_eval<KIND, NF, PHYS, MODE>(args, e, valid, lds, work,
+                                     [&](int w, float v) { args.Rel[e * NDOF + w] = v; }, bad, threadIdx.x);
+  if constexpr (DO_K) {
+    // the whole wavefront copies its elements' contiguous storage range with consecutive lanes on consecutive words
     constexpr int EW = T::NPAIRS * NF * NF;   // words per element
     constexpr int EPWAVE = 64 / G;            // elements per wavefront
     const int wl = threadIdx.x % 64, wv = threadIdx.x / 64;
@@ -638,6 +657,284 @@ static void launch_modes(int mode, const ElementArgs& a, hipStream_t s) {
       break;
     default: break;
   }
+}
+
+// ------------------------------------------------------------------ fused assembly of structured hex8 meshes
+// On a structured hex8 mesh (StructuredMesh numbering, src/meshes/StructuredMesh.jl:85-131) the residual and the CSR
+// Jacobian are assembled in one kernel, without the element-private storage of the two-phase path and its round trip
+// through HBM (written once by the element kernel, gathered again by the reduction).
+//
+// A work-group owns FT_X x FT_Y node columns and a range [z0, z1) of node planes, and marches up through the element
+// layers k = z0 - 1 .. z1 - 1. For each layer it evaluates the (FT_X + 1) x (FT_Y + 1) = 32 elements around its columns
+// (element_eval, the same code and bits as the element kernel) and leaves their blocks and rows in LDS. Node plane k
+// touches layers k - 1 and k only, so after layer k:
+//   - the rows of plane k are complete: each block is its partial sum over layer k - 1, kept in FP64 registers since
+//     the previous layer, plus its blocks from layer k; they are written to the CSR values, each entry once;
+//   - the layer-k blocks of plane k + 1 become the new partial sums (only neighbours one plane up or level with a
+//     node have any: 18 of the 27 blocks of a row).
+// Elements of the halo ring around the columns and of the layer below z0 are evaluated by both work-groups that need
+// them (32 elements per 21 columns). The FP64 sum of an entry runs over its element contributions of layer k - 1 in
+// ascending element id, then over those of layer k: the contributions of the two-phase path, grouped by layer.
+// Column slots come from a per-node mask of the 27 neighbours present in its CSR row (existing and free): neighbour
+// offsets (dx, dy, dz) in lexicographic (dz, dy, dx) order are the row's ascending node order.
+#ifndef FUSED_WAVES
+#define FUSED_WAVES 2
+#endif
+constexpr int FT_X = 7, FT_Y = 3, FT_NODES = FT_X * FT_Y;
+constexpr int FT_PBLK = FT_NODES * 18;             // blocks with a partial sum (dz = -1, 0)
+constexpr int FT_BLK = FT_NODES * 27;              // blocks of a plane
+static_assert((FT_X + 1) * (FT_Y + 1) == ELEM_THREADS / 8, "one element layer per work-group pass");
+static_assert(FT_PBLK <= 2 * ELEM_THREADS && FT_BLK <= 3 * ELEM_THREADS, "at most 2 partial and 3 blocks per lane");
+
+__device__ __forceinline__ int hex_local(int ox, int oy, int oz) {  // StructuredMesh.jl:116-123 local order
+  return oz * 4 + (oy ? (ox ? 2 : 3) : (ox ? 1 : 0));
+}
+
+template <int NF, int PHYS>
+__global__ __launch_bounds__(ELEM_THREADS) __attribute__((amdgpu_waves_per_eu(FUSED_WAVES, 8)))
+void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
+  using LDS = ElemLDS<FECX_HEX8, NF, PHYS, MODE_R | MODE_K>;
+  constexpr int EPW = ELEM_THREADS / 8, NB = NF * NF, KW = LDS::KW;
+  __shared__ LDS lds[EPW];
+  __shared__ __attribute__((aligned(16))) ElemWork<LDS, EPW, 8> work;
+  __shared__ float srow[EPW][8 * NF];  // the layer's element rows (residual)
+  const int tid = threadIdx.x, grp = tid / 8;
+  const int64_t wg = blockIdx.x;
+  const int tx = (int)(wg % f.tiles_x), ty = (int)((wg / f.tiles_x) % f.tiles_y), tz = (int)(wg / f.tiles_x / f.tiles_y);
+  const int x0 = tx * FT_X, y0 = ty * FT_Y;
+  const int z0 = (int)((int64_t)f.nz * tz / f.tiles_z), z1 = (int)((int64_t)f.nz * (tz + 1) / f.tiles_z);
+  const int nex = f.nx - 1, ney = f.ny - 1, nez = f.nz - 1;
+  // this lane's element slot: ex = x0 - 1 + exl, ey = y0 - 1 + eyl
+  const int exl = grp >> 2, eyl = grp & 3;
+  const int ex = x0 - 1 + exl, ey = y0 - 1 + eyl;
+  const bool col_ok = ex >= 0 && ex < nex && ey >= 0 && ey < ney;
+  // this lane's blocks: t = 0 (and t = 1 for tid < FT_PBLK - 256) hold a partial sum; the others (dz = +1) do not
+  int bn[3], bd[3];  // node (li + FT_X lj) and neighbour d of block t, or -1
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const int b = tid + ELEM_THREADS * t;
+    if (b < FT_PBLK) { bn[t] = b / 18; bd[t] = b % 18; }
+    else if (b < FT_BLK) { bn[t] = (b - FT_PBLK) / 9; bd[t] = 18 + (b - FT_PBLK) % 9; }
+    else { bn[t] = -1; bd[t] = 0; }
+  }
+  double part[2][NB];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int c = 0; c < NB; ++c) part[t][c] = 0.0;
+  double rpart = 0.0;  // residual lanes: tid < FT_NODES * NF, node tid / NF, component tid % NF
+  bool bad = false;
+
+  // sum over the layer's elements holding node (li, lj) at height oz and its neighbour d: the blocks K_ab, in
+  // ascending element id (ex, then ey)
+  auto layer_block = [&](int li, int lj, int d, int oz, double (&acc)[NB]) {
+    const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+    const int mz = oz + dz;
+    if (mz < 0 || mz > 1) return;
+#pragma unroll 1
+    for (int cc = 0; cc < 4; ++cc) {
+        const int cx = cc >> 1, cy = cc & 1;
+        const int ox = 1 - cx, oy = 1 - cy;  // node's offsets in element slot (li + cx, lj + cy)
+        const int mx = ox + dx, my = oy + dy;
+        const int sx = li + cx, sy = lj + cy, e_x = x0 - 1 + sx, e_y = y0 - 1 + sy;
+        if (mx < 0 || mx > 1 || my < 0 || my > 1 || e_x < 0 || e_x >= nex || e_y < 0 || e_y >= ney) continue;
+        const PairRef pr = pair_ref(8, hex_local(ox, oy, oz), hex_local(mx, my, mz));
+        const float* blk = work.K + (sx * 4 + sy) * KW + pr.slot * NB;
+#pragma unroll
+        for (int i = 0; i < NF; ++i)
+#pragma unroll
+          for (int k = 0; k < NF; ++k) acc[i * NF + k] += (double)blk[pr.ab ? i * NF + k : k * NF + i];
+      }
+  };
+  auto layer_row = [&](int li, int lj, int comp, int oz) {
+    double s = 0.0;
+#pragma unroll
+    for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy) {
+        const int sx = li + cx, sy = lj + cy, e_x = x0 - 1 + sx, e_y = y0 - 1 + sy;
+        if (e_x < 0 || e_x >= nex || e_y < 0 || e_y >= ney) continue;
+        s += (double)srow[sx * 4 + sy][hex_local(1 - cx, 1 - cy, oz) * NF + comp];
+      }
+    return s;
+  };
+
+  for (int k = z0 - 1; k < z1; ++k) {
+    const bool layer_ok = k >= 0 && k < nez;
+    const bool valid = layer_ok && col_ok;
+    const int64_t eid = valid ? ((int64_t)ex * ney + ey) * nez + k : 0;
+    const int64_t e = valid ? (int64_t)f.epos[eid] : 0;
+    // the thread index through an empty asm statement: the element evaluation's per-lane constants (quadrature point,
+    // shape functions) are then recomputed for every layer instead of held in registers across the loop
+    int tid_k = tid;
+    asm volatile("" : "+v"(tid_k));
+    element_eval<FECX_HEX8, NF, PHYS, MODE_R | MODE_K>(args, e, valid, lds, work,
+                                                         [&](int w, float v) { srow[grp][w] = v; }, bad, tid_k);
+    // plane k: complete its rows and write them; plane k + 1: the new partial sums
+    const bool write = k >= z0;
+    // block t of this lane: its plane-k value starts from the partial sum `p` (blocks d < 18) or zero
+    auto finish_block = [&](int t, const double* p) {
+      const int li = bn[t] % FT_X, lj = bn[t] / FT_X, d = bd[t];
+      double acc[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) acc[c] = p ? p[c] : 0.0;
+      if (layer_ok) layer_block(li, lj, d, 0, acc);
+      const int gx = x0 + li, gy = y0 + lj;
+      if (write && gx < f.nx && gy < f.ny && f.vals) {
+        const int64_t n = gx + (int64_t)f.nx * (gy + (int64_t)f.ny * k);
+        const int2 info = f.nodeinfo[n];
+        const uint32_t mask = (uint32_t)info.y;
+        if ((mask >> d) & 1u) {
+          const int deg = __popc(mask), slot = __popc(mask & ((1u << d) - 1u));
+          double* dst = f.vals + (int64_t)NB * info.x + NF * slot;
+#pragma unroll
+          for (int i = 0; i < NF; ++i)
+#pragma unroll
+            for (int kk = 0; kk < NF; ++kk)
+              __builtin_nontemporal_store(acc[i * NF + kk], dst + (int64_t)i * NF * deg + kk);
+        }
+      }
+    };
+    auto next_partial = [&](int t, double (&p)[NB]) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) p[c] = 0.0;
+      if (layer_ok) layer_block(bn[t] % FT_X, bn[t] / FT_X, bd[t], 1, p);
+    };
+    finish_block(0, part[0]);  // block 0 of every lane has a partial sum (tid < FT_PBLK)
+    next_partial(0, part[0]);
+    if (bn[1] >= 0) {
+      if (bd[1] < 18) {
+        finish_block(1, part[1]);
+        next_partial(1, part[1]);
+      } else {
+        finish_block(1, nullptr);
+      }
+    }
+    if (bn[2] >= 0) finish_block(2, nullptr);
+    if (tid < FT_NODES * NF) {
+      const int nl = tid / NF, comp = tid % NF, li = nl % FT_X, lj = nl / FT_X;
+      double r = rpart;
+      if (layer_ok) r += layer_row(li, lj, comp, 0);
+      const int gx = x0 + li, gy = y0 + lj;
+      if (write && gx < f.nx && gy < f.ny && f.R) f.R[(gx + (int64_t)f.nx * (gy + (int64_t)f.ny * k)) * NF + comp] = r;
+      rpart = layer_ok ? layer_row(li, lj, comp, 1) : 0.0;
+    }
+    __syncthreads();  // the next layer's element_eval overwrites lds, work and srow
+  }
+  if (bad) atomicOr(args.flag, 1);
+}
+
+// The two scatter designs the fused kernel is measured against, on the same structured hex8 meshes and the same
+// element code: every element's blocks and rows are added straight into the CSR values and the residual, which are
+// zero-filled first. ATOMIC = 1: FP64 atomic adds (global_atomic_add_f64), all elements in one launch, each lane
+// adding the 8 blocks of its element row (wavefronts hold 8 elements whose rows share nodes, so each entry still
+// takes several atomics). ATOMIC = 0: element colouring -- 8 launches, one per colour (ex, ey, ez) mod 2, whose
+// elements share no node, so plain read-add-write stores do not race. Summation order: atomics in arrival order
+// (not deterministic), colouring in colour order.
+template <int NF, int PHYS, int ATOMIC>
+__global__ __launch_bounds__(ELEM_THREADS) __attribute__((amdgpu_waves_per_eu(2, 8)))
+void scatter_hex8_kernel(ElementArgs args, FusedArgs f, int colour) {
+  using LDS = ElemLDS<FECX_HEX8, NF, PHYS, MODE_R | MODE_K>;
+  constexpr int EPW = ELEM_THREADS / 8, NB = NF * NF, KW = LDS::KW;
+  __shared__ LDS lds[EPW];
+  __shared__ __attribute__((aligned(16))) ElemWork<LDS, EPW, 8> work;
+  __shared__ float srow[EPW][8 * NF];
+  const int grp = threadIdx.x / 8, a = threadIdx.x % 8;
+  const int nex = f.nx - 1, ney = f.ny - 1, nez = f.nz - 1;
+  // element of this lane group: all elements in id order (atomics), or the colour's sub-grid
+  int64_t idx = (int64_t)blockIdx.x * EPW + grp;
+  int ex, ey, ez;
+  bool valid;
+  if (ATOMIC) {
+    valid = idx < (int64_t)nex * ney * nez;
+    ez = (int)(idx % nez); ey = (int)((idx / nez) % ney); ex = (int)(idx / ((int64_t)nez * ney));
+  } else {
+    const int cx = colour & 1, cy = (colour >> 1) & 1, cz = (colour >> 2) & 1;
+    const int mx = (nex - cx + 1) / 2, my = (ney - cy + 1) / 2, mz = (nez - cz + 1) / 2;
+    valid = mx > 0 && my > 0 && mz > 0 && idx < (int64_t)mx * my * mz;
+    const int iz = mz > 0 ? (int)(idx % mz) : 0, iy = my > 0 ? (int)((idx / mz) % my) : 0;
+    const int ix = (mz > 0 && my > 0) ? (int)(idx / ((int64_t)mz * my)) : 0;
+    ex = 2 * ix + cx; ey = 2 * iy + cy; ez = 2 * iz + cz;
+  }
+  if (!valid) { ex = ey = ez = 0; }
+  const int64_t e = valid ? (int64_t)f.epos[((int64_t)ex * ney + ey) * nez + ez] : 0;
+  bool bad = false;
+  element_eval<FECX_HEX8, NF, PHYS, MODE_R | MODE_K>(args, e, valid, lds, work,
+                                                       [&](int w, float v) { srow[grp][w] = v; }, bad, threadIdx.x);
+  if (valid) {
+    static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    const int64_t na = (ex + off[a][0]) + (int64_t)f.nx * ((ey + off[a][1]) + (int64_t)f.ny * (ez + off[a][2]));
+    if (f.R) {
+#pragma unroll
+      for (int i = 0; i < NF; ++i) {
+        double* r = f.R + na * NF + i;
+        const double v = (double)srow[grp][a * NF + i];
+        if (ATOMIC) atomicAdd(r, v);
+        else *r += v;
+      }
+    }
+    const int2 info = f.nodeinfo[na];
+    const uint32_t mask = (uint32_t)info.y;
+    const int deg = __popc(mask);
+    for (int b = 0; b < 8; ++b) {
+      const int d = (off[b][2] - off[a][2] + 1) * 9 + (off[b][1] - off[a][1] + 1) * 3 + (off[b][0] - off[a][0] + 1);
+      if (!((mask >> d) & 1u)) continue;
+      const int slot = __popc(mask & ((1u << d) - 1u));
+      const PairRef pr = pair_ref(8, a, b);
+      const float* blk = work.K + grp * KW + pr.slot * NB;
+      double* dst = f.vals + (int64_t)NB * info.x + NF * slot;
+#pragma unroll
+      for (int i = 0; i < NF; ++i)
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+          double* p = dst + (int64_t)i * NF * deg + k;
+          const double v = (double)blk[pr.ab ? i * NF + k : k * NF + i];
+          if (ATOMIC) atomicAdd(p, v);
+          else *p += v;
+        }
+    }
+  }
+  if (bad) atomicOr(args.flag, 1);
+}
+
+// scatter 1: FP64 atomics, 2: element colouring; vals and R zero-filled by the caller
+int launch_scatter_hex8(int nf, int scatter, const ElementArgs& a, const FusedArgs& f, hipStream_t s) {
+  const int phys = a.mp.phys;
+  const int64_t nex = f.nx - 1, ney = f.ny - 1, nez = f.nz - 1;
+  constexpr int EPW = ELEM_THREADS / 8;
+  auto go = [&](auto kern) {
+    if (scatter == 1) {
+      kern<<<(unsigned)((nex * ney * nez + EPW - 1) / EPW), ELEM_THREADS, 0, s>>>(a, f, 0);
+    } else {
+      for (int c = 0; c < 8; ++c) {
+        const int64_t m = ((nex - (c & 1) + 1) / 2) * ((ney - ((c >> 1) & 1) + 1) / 2) * ((nez - ((c >> 2) & 1) + 1) / 2);
+        if (m > 0) kern<<<(unsigned)((m + EPW - 1) / EPW), ELEM_THREADS, 0, s>>>(a, f, c);
+      }
+    }
+  };
+  if (scatter != 1 && scatter != 2) return -1;
+  if (nf == 1 && phys == FECX_POISSON) {
+    if (scatter == 1) go(scatter_hex8_kernel<1, FECX_POISSON, 1>); else go(scatter_hex8_kernel<1, FECX_POISSON, 0>);
+    return 0;
+  }
+  if (nf == 3 && phys == FECX_NEOHOOKEAN) {
+    if (scatter == 1) go(scatter_hex8_kernel<3, FECX_NEOHOOKEAN, 1>); else go(scatter_hex8_kernel<3, FECX_NEOHOOKEAN, 0>);
+    return 0;
+  }
+  return -1;
+}
+
+int launch_fused_hex8(int nf, const ElementArgs& a, const FusedArgs& f, hipStream_t s) {
+  const unsigned grid = (unsigned)((int64_t)f.tiles_x * f.tiles_y * f.tiles_z);
+  const int phys = a.mp.phys;
+  if (nf == 1 && phys == FECX_POISSON) { fused_hex8_kernel<1, FECX_POISSON><<<grid, ELEM_THREADS, 0, s>>>(a, f); return 0; }
+  if (nf == 3 && phys == FECX_LINEAR_ELASTIC) {
+    fused_hex8_kernel<3, FECX_LINEAR_ELASTIC><<<grid, ELEM_THREADS, 0, s>>>(a, f);
+    return 0;
+  }
+  if (nf == 3 && phys == FECX_NEOHOOKEAN) { fused_hex8_kernel<3, FECX_NEOHOOKEAN><<<grid, ELEM_THREADS, 0, s>>>(a, f); return 0; }
+  if (nf == 3 && phys == FECX_J2) { fused_hex8_kernel<3, FECX_J2><<<grid, ELEM_THREADS, 0, s>>>(a, f); return 0; }
+  return -1;
 }
 
 int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t s) {

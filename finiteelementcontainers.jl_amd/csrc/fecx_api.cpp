@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 
 #include "api_util.h"
 #include "surrogate.h"
@@ -17,7 +18,7 @@ struct Prof {
   explicit Prof(fecx_ctx* c) : ctx(c) {}
   template <class F>
   void run(const char* name, hipStream_t st, F f) {
-    if (!ctx->profiling) { f(); return; }
+    if (!ctx || !ctx->profiling) { f(); return; }
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
@@ -27,6 +28,7 @@ struct Prof {
     ev.push_back({name, {a, b}});
   }
   void finish() {
+    if (!ctx) return;
     for (auto& x : ev) {
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, x.second.first, x.second.second));
@@ -102,12 +104,80 @@ int pipeline_chunks(fecx_ctx* ctx, int mode) {
   return ctx->chunks > 0 ? ctx->chunks : 1;
 }
 
+// A hex8 mesh numbered as StructuredMesh numbers it (src/meshes/StructuredMesh.jl:85-131): node (i, j, k) is
+// i + nx (j + ny k) (0-based), element (ex, ey, ez) is ez + nez (ey + ney ex) with local nodes in the order of
+// StructuredMesh.jl:116-123. Finds the node counts from the first element and checks every element against them.
+bool detect_structured_hex8(fecx_ctx* ctx) {
+  ctx->structured = false;
+  if (ctx->kind != FECX_HEX8 || ctx->block_start.size() != 2 || ctx->ne < 1) return false;
+  const int32_t* c = ctx->conn0.data();
+  if (c[0] != 0 || c[1] != 1) return false;
+  const int64_t nx = c[3], nxy = c[4];
+  if (nx < 2 || nxy < 2 * nx || nxy % nx != 0 || ctx->nn % nxy != 0) return false;
+  const int64_t ny = nxy / nx, nz = ctx->nn / nxy;
+  if (nz < 2 || (nx - 1) * (ny - 1) * (nz - 1) != ctx->ne || nx >= (1 << 20) || ny >= (1 << 20)) return false;
+  const int64_t nex = nx - 1, ney = ny - 1, nez = nz - 1;
+  static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  for (int64_t e = 0; e < ctx->ne; ++e) {
+    const int64_t ez = e % nez, ey = (e / nez) % ney, ex = e / (nez * ney);
+    for (int a = 0; a < 8; ++a) {
+      const int64_t n = (ex + off[a][0]) + nx * ((ey + off[a][1]) + ny * (ez + off[a][2]));
+      if (c[e * 8 + a] != n) return false;
+    }
+  }
+  ctx->snx = (int)nx; ctx->sny = (int)ny; ctx->snz = (int)nz;
+  ctx->structured = true;
+  return true;
+}
+
+// Per node {first CSR block of its row, mask of the 27 neighbours (dx, dy, dz) present in the row}, bit
+// (dz + 1) 9 + (dy + 1) 3 + (dx + 1): the neighbour exists and both nodes are free. Checked against the pattern's
+// row degrees; returns false (and the fused path stays off) on a mismatch.
+bool build_nodeinfo(fecx_ctx* ctx) {
+  const auto& hp = ctx->hp;
+  const int64_t nx = ctx->snx, ny = ctx->sny, nz = ctx->snz;
+  std::vector<int32_t> info((size_t)ctx->nn * 2, 0);
+  for (int64_t n = 0; n < ctx->nn; ++n) {
+    const int32_t r = hp.free_rank[n];
+    if (r < 0) continue;
+    const int64_t i = n % nx, j = (n / nx) % ny, k = n / (nx * ny);
+    uint32_t mask = 0;
+    for (int d = 0; d < 27; ++d) {
+      const int64_t mi = i + d % 3 - 1, mj = j + (d / 3) % 3 - 1, mk = k + d / 9 - 1;
+      if (mi < 0 || mi >= nx || mj < 0 || mj >= ny || mk < 0 || mk >= nz) continue;
+      if (hp.free_rank[mi + nx * (mj + ny * mk)] >= 0) mask |= 1u << d;
+    }
+    if ((int64_t)__builtin_popcount(mask) != hp.bptr[r + 1] - hp.bptr[r]) return false;
+    if (hp.bptr[r] >= (int64_t(1) << 31)) return false;
+    info[2 * n] = (int32_t)hp.bptr[r];
+    info[2 * n + 1] = (int32_t)mask;
+  }
+  upload(ctx, "nodeinfo", info);
+  return true;
+}
+
+// The scatter design a matrix assembly takes (FECX_SCATTER_*): the fused kernel for residual + Jacobian (or Jacobian)
+// assemblies of a one-block structured hex8 mesh with a plain material, unless another design is asked for; the
+// atomic and coloured designs only where they are built (structured hex8, Poisson or neo-Hookean); the two-phase path
+// everywhere else.
+int scatter_path(fecx_ctx* ctx, int mode, bool surrogate) {
+  if (!ctx->structured || surrogate || ctx->block_mp.size() != 1) return FECX_SCATTER_TWO_PHASE;
+  if (mode != (fecx::MODE_R | fecx::MODE_K) && mode != fecx::MODE_K) return FECX_SCATTER_TWO_PHASE;
+  const int ph = ctx->mp.phys;
+  const bool fused_ok = (ctx->nf == 1 && ph == FECX_POISSON) ||
+                        (ctx->nf == 3 && (ph == FECX_LINEAR_ELASTIC || ph == FECX_NEOHOOKEAN || ph == FECX_J2));
+  const bool direct_ok = (ctx->nf == 1 && ph == FECX_POISSON) || (ctx->nf == 3 && ph == FECX_NEOHOOKEAN);
+  if (ctx->scatter == FECX_SCATTER_FUSED && fused_ok) return FECX_SCATTER_FUSED;
+  if ((ctx->scatter == FECX_SCATTER_ATOMIC || ctx->scatter == FECX_SCATTER_COLOURED) && direct_ok) return ctx->scatter;
+  return FECX_SCATTER_TWO_PHASE;
+}
+
 // The numeric flag before a launch: cleared on the device only if the last read-back did not find it clear.
 int* arm_flag(fecx_ctx* ctx) {
   int* flag = get<int>(ctx, "flag");
   if (ctx->flag_dirty) zero_int_kernel<<<1, 1, 0, ctx->stream>>>(flag);
   ctx->flag_dirty = true;  // until this call reads it back as clear
-  if (!ctx->flag_host) HIPCHK(hipHostMalloc((void**)&ctx->flag_host, sizeof(int), hipHostMallocDefault));
+  require(ctx->flag_host != nullptr, FECX_ERR_STATE, "no host flag word");
   return flag;
 }
 
@@ -130,11 +200,14 @@ int read_flag(fecx_ctx* ctx, const int* flag) {
 void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double* R, double* vals) {
   check_ready(ctx);
   require(U != nullptr, FECX_ERR_ARG, "U is NULL");
+  const bool surrogate_phys = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
+  const bool surrogate = surrogate_phys && ctx->sg_enabled;
+  float *kel = get<float>(ctx, "Kel"), *rel = get<float>(ctx, "Rel");
   Prof prof(ctx);
   int* flag = arm_flag(ctx);
   fecx::ElementArgs ea = element_args(ctx, U, V);
-  const bool surrogate_phys = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
-  const bool surrogate = surrogate_phys && ctx->sg_enabled;
+  ea.Kel = kel;
+  ea.Rel = rel;
   if (surrogate_phys && !surrogate) ea.mp.phys = ctx->mp.phys == FECX_J2_SURROGATE ? FECX_J2 : FECX_NEOHOOKEAN;
   if (ctx->ns > 0) require(ea.st_old != nullptr, FECX_ERR_STATE, "state not allocated");
   if (surrogate) {
@@ -151,7 +224,40 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
     prof.run("surrogate_material", ctx->stream, [&] { fecx::surrogate_qp_pass(ctx, ea.hq, (float*)ea.qpmat, ctx->stream); });
     HIPCHK(hipGetLastError());
   }
+  const int path = scatter_path(ctx, mode, surrogate_phys);
+  if (path != FECX_SCATTER_TWO_PHASE) {
+    fecx::FusedArgs fa{};
+    fa.nx = ctx->snx; fa.ny = ctx->sny; fa.nz = ctx->snz;
+    fa.tiles_x = (fa.nx + fecx::FUSED_TILE_X - 1) / fecx::FUSED_TILE_X;
+    fa.tiles_y = (fa.ny + fecx::FUSED_TILE_Y - 1) / fecx::FUSED_TILE_Y;
+    // z ranges of about 16 node planes, at least enough work-groups for three per CU
+    fa.tiles_z = std::max(1, std::min(fa.nz, std::max((fa.nz + 15) / 16, (768 + fa.tiles_x * fa.tiles_y - 1) /
+                                                                           (fa.tiles_x * fa.tiles_y))));
+    fa.epos = get<int32_t>(ctx, "epos");
+    fa.nodeinfo = get<int2>(ctx, "nodeinfo");
+    fa.vals = (mode & fecx::MODE_K) ? vals : nullptr;
+    fa.R = (mode & fecx::MODE_R) ? R : nullptr;
+    int rc = 0;
+    if (path == FECX_SCATTER_FUSED) {
+      prof.run("fused", ctx->stream, [&] { rc = fecx::launch_fused_hex8(ctx->nf, ea, fa, ctx->stream); });
+    } else {  // atomic or coloured: zero-fill, then every element adds its blocks and rows in place
+      const size_t rbytes = sizeof(double) * (size_t)ctx->nn * ctx->nf;
+      prof.run(path == FECX_SCATTER_ATOMIC ? "atomic" : "coloured", ctx->stream, [&] {
+        if (fa.vals) HIPCHK(hipMemsetAsync(fa.vals, 0, sizeof(double) * (size_t)ctx->nnz, ctx->stream));
+        if (fa.R) HIPCHK(hipMemsetAsync(fa.R, 0, rbytes, ctx->stream));
+        rc = fecx::launch_scatter_hex8(ctx->nf, path == FECX_SCATTER_ATOMIC ? 1 : 2, ea, fa, ctx->stream);
+      });
+    }
+    require(rc == 0, FECX_ERR_UNSUPPORTED, "no kernel of this scatter design for this nf / physics");
+    HIPCHK(hipGetLastError());
+    const int hflag = read_flag(ctx, flag);
+    prof.finish();
+    require(hflag == 0, FECX_ERR_NUMERIC, "element kernel met a non-positive Jacobian determinant or a NaN");
+    return;
+  }
   fecx::ReduceArgs ra = reduce_args(ctx);
+  ra.Kel = kel;
+  ra.Rel = rel;
   if (mode & (fecx::MODE_R | fecx::MODE_ACTION_MF | fecx::MODE_ACTION_ASM | fecx::MODE_DIAG)) ra.R = R;
   if (mode & fecx::MODE_K) ra.vals = vals;
   const int nblocks = (int)ctx->block_mp.size();
@@ -241,7 +347,10 @@ int fecx_create(fecx_ctx** out, int device) {
   }
   fecx_ctx* ctx = new fecx_ctx();
   ctx->device = device;
-  int rc = guard(ctx, [&] { (void)dev(ctx, "flag", sizeof(int)); });
+  int rc = guard(ctx, [&] {
+    (void)dev(ctx, "flag", sizeof(int));
+    if (device >= 0) HIPCHK(hipHostMalloc((void**)&ctx->flag_host, sizeof(int), hipHostMallocDefault));
+  });
   if (rc) { delete ctx; return rc; }
   *out = ctx;
   return FECX_OK;
@@ -438,6 +547,7 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
     }
     dev(ctx, "Rel", sizeof(float) * ctx->ne * ctx->nnpe * nf);
     dev(ctx, "Kel", sizeof(float) * ctx->ne * ctx->pairs.npairs * nf * nf);
+    if (detect_structured_hex8(ctx) && ctx->device >= 0 && !build_nodeinfo(ctx)) ctx->structured = false;
     ctx->has_pattern = true;
     if (nnz) *nnz = ctx->nnz;
   });
@@ -686,6 +796,22 @@ int fecx_extract_field_unknowns(fecx_ctx* ctx, const double* U, double* Uu) {
     fecx::launch_gather_unknowns((int64_t)ctx->unknown_dofs.size(), get<int64_t>(ctx, "unknown_idx"), U, Uu,
                                  ctx->stream);
     HIPCHK(hipGetLastError());
+  });
+}
+
+int fecx_set_scatter(fecx_ctx* ctx, int design) {
+  return guard(ctx, [&] {
+    require(design >= FECX_SCATTER_FUSED && design <= FECX_SCATTER_COLOURED, FECX_ERR_ARG, "unknown scatter design");
+    ctx->scatter = design;
+  });
+}
+
+int fecx_get_scatter_path(fecx_ctx* ctx, int* path) {
+  return guard(ctx, [&] {
+    require(path != nullptr, FECX_ERR_ARG, "path is NULL");
+    require(ctx->has_pattern && ctx->has_material, FECX_ERR_STATE, "pattern and material must be set");
+    const bool surrogate = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
+    *path = scatter_path(ctx, fecx::MODE_R | fecx::MODE_K, surrogate);
   });
 }
 

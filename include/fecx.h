@@ -329,6 +329,26 @@ int fecx_j2_generate_samples(fecx_ctx* ctx, int64_t npaths, int steps, float max
  * sums, same order). */
 int fecx_set_chunks(fecx_ctx* ctx, int chunks);
 
+/* Scatter design of matrix assemblies (no counterpart in the reference, whose element loop stores element-private
+ * matrices and sparse! sums them afterwards, src/assemblers/Matrix.jl:35-75, SparsityPatterns.jl:301-329):
+ *   FECX_SCATTER_FUSED      (default) on a one-block hex8 mesh numbered as StructuredMesh numbers it
+ *                           (src/meshes/StructuredMesh.jl:85-131; detected by fecx_build_pattern), residual + Jacobian
+ *                           and Jacobian-only assemblies with a plain material run as one kernel that sums each CSR
+ *                           row from the element matrices in on-chip memory; the two-phase path elsewhere
+ *   FECX_SCATTER_TWO_PHASE  element kernel into element-private storage, then the FP64 owner-computes reduction
+ *   FECX_SCATTER_ATOMIC     zero fill, then FP64 atomic adds of every element block (structured hex8, Poisson and
+ *                           neo-Hookean; the two-phase path elsewhere); summation order not deterministic
+ *   FECX_SCATTER_COLOURED   zero fill, then 8 element colours whose elements share no node, plain adds
+ *                           (same meshes as atomic)
+ * The designs sum the same FP32 element contributions in FP64 in different orders, so they agree to FP64 rounding.
+ * fecx_get_scatter_path reports the design a residual + Jacobian call of the context takes. */
+#define FECX_SCATTER_FUSED 0
+#define FECX_SCATTER_TWO_PHASE 1
+#define FECX_SCATTER_ATOMIC 2
+#define FECX_SCATTER_COLOURED 3
+int fecx_set_scatter(fecx_ctx* ctx, int design);
+int fecx_get_scatter_path(fecx_ctx* ctx, int* path);
+
 /* When on, every kernel launch is bracketed by HIP events on the stream it runs on and its
  * duration accumulated per kernel name (the reference's only instrumentation is TimerOutputs
  * sections, src/Solvers.jl:133-152). */
@@ -343,6 +363,13 @@ int fecx_profile_entry(fecx_ctx* ctx, int i, char* name, int name_len, int64_t* 
 
 /* Clears the accumulated profile. */
 int fecx_profile_reset(fecx_ctx* ctx);
+
+/* HBM bandwidth ceiling of the context's device, for the rooflines the assembly kernels are priced against (no
+ * counterpart in the reference): the average milliseconds of one streaming pass over the device buffer `buf` of
+ * `bytes` (a multiple of 32, at least 4096) by a hand-written kernel, over `reps` passes after one warm-up, timed with
+ * HIP events on the context's stream. op 0: read every byte; 1: write every byte; 2: copy the first half of the
+ * buffer to the second (bytes / 2 read and bytes / 2 written). */
+int fecx_measure_bandwidth(fecx_ctx* ctx, int op, void* buf, int64_t bytes, int reps, double* ms);
 
 #ifdef __cplusplus
 }

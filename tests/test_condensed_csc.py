@@ -179,7 +179,11 @@ def test_condensed_matches_oracle(fec, orc, gpu, case, sp_type):
                               c)
     rp, cv, vals = _full_csr(orc, wl, _storage(orc, wl, U))
     K_ref = orc.adjust_matrix(rp, cv, vals, c)
-    KV_ref = orc.assemble_action(k, wl.mesh.coords, wl.mesh.conn, wl.nf, wl.physics.kind, props, U, Vf)
+    # the scratch field V holds Vu at the unknown dofs only (update_field_unknowns! writes those), zero at the
+    # Dirichlet dofs; hvp(asm, v) then sets the constrained entries to v
+    Vfield = Vf.copy()
+    Vfield[dd - 1] = 0.0
+    KV_ref = orc.assemble_action(k, wl.mesh.coords, wl.mesh.conn, wl.nf, wl.physics.kind, props, U, Vfield)
 
     fec.assemble_vector_(asm, fec.residual, Uu, p)
     R = fec.residual(asm).cpu().numpy().copy()
