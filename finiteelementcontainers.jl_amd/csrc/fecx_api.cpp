@@ -159,9 +159,9 @@ bool build_nodeinfo(fecx_ctx* ctx) {
 // The scatter design a matrix assembly takes (FECX_SCATTER_*): the fused kernel for residual + Jacobian (or Jacobian)
 // assemblies of a one-block structured hex8 mesh with a plain material, unless another design is asked for; the
 // atomic and coloured designs only where they are built (structured hex8, Poisson or neo-Hookean); the two-phase path
-// everywhere else.
-int scatter_path(fecx_ctx* ctx, int mode, bool surrogate) {
-  if (!ctx->structured || surrogate || ctx->block_mp.size() != 1) return FECX_SCATTER_TWO_PHASE;
+// everywhere else, and always for the allocation-free entry point.
+int scatter_path(fecx_ctx* ctx, int mode, bool surrogate, bool noalloc) {
+  if (!ctx->structured || noalloc || surrogate || ctx->block_mp.size() != 1) return FECX_SCATTER_TWO_PHASE;
   if (mode != (fecx::MODE_R | fecx::MODE_K) && mode != fecx::MODE_K) return FECX_SCATTER_TWO_PHASE;
   const int ph = ctx->mp.phys;
   const bool fused_ok = (ctx->nf == 1 && ph == FECX_POISSON) ||
@@ -197,13 +197,30 @@ int read_flag(fecx_ctx* ctx, const int* flag) {
 // element chunks 0..j: element chunks run in order on stream2, reduction chunk j on the context's stream after an
 // event recorded behind element chunk j. Element chunk j + 1 (arithmetic-bound) overlaps reduction chunk j
 // (memory-bound), which reads element storage written moments earlier.
-void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double* R, double* vals) {
+// With `noalloc`, the element-private outputs go to the per-device scratch array (noalloc.hip) instead of the
+// context's buffers, and nothing is allocated: no profiling events, one piece, no surrogate pass.
+void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double* R, double* vals,
+              bool noalloc = false) {
   check_ready(ctx);
   require(U != nullptr, FECX_ERR_ARG, "U is NULL");
   const bool surrogate_phys = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
   const bool surrogate = surrogate_phys && ctx->sg_enabled;
   float *kel = get<float>(ctx, "Kel"), *rel = get<float>(ctx, "Rel");
+  if (noalloc) {
+    require(!surrogate, FECX_ERR_UNSUPPORTED, "the allocation-free entry point does not run surrogate materials");
+    const size_t kb = sizeof(float) * (size_t)ctx->ne * ctx->pairs.npairs * ctx->nf * ctx->nf;
+    const size_t rb = sizeof(float) * (size_t)ctx->ne * ctx->nnpe * ctx->nf;
+    const size_t kb_al = (kb + 255) / 256 * 256;
+    require(kb_al + rb <= fecx::NOALLOC_SCRATCH_BYTES, FECX_ERR_ARG,
+            "the allocation-free entry point needs " + std::to_string(kb_al + rb) + " bytes of element scratch; it "
+            "holds at most " + std::to_string(fecx::NOALLOC_SCRATCH_BYTES) + " (use fecx_assemble_vector_matrix)");
+    unsigned char* base = (unsigned char*)fecx::noalloc_scratch(ctx->device);
+    require(base != nullptr, FECX_ERR_STATE, "no scratch address for this device");
+    kel = (float*)base;
+    rel = (float*)(base + kb_al);
+  }
   Prof prof(ctx);
+  if (noalloc) prof.ctx = nullptr;  // no HIP events
   int* flag = arm_flag(ctx);
   fecx::ElementArgs ea = element_args(ctx, U, V);
   ea.Kel = kel;
@@ -224,7 +241,7 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
     prof.run("surrogate_material", ctx->stream, [&] { fecx::surrogate_qp_pass(ctx, ea.hq, (float*)ea.qpmat, ctx->stream); });
     HIPCHK(hipGetLastError());
   }
-  const int path = scatter_path(ctx, mode, surrogate_phys);
+  const int path = scatter_path(ctx, mode, surrogate_phys, noalloc);
   if (path != FECX_SCATTER_TWO_PHASE) {
     fecx::FusedArgs fa{};
     fa.nx = ctx->snx; fa.ny = ctx->sny; fa.nz = ctx->snz;
@@ -261,7 +278,7 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   if (mode & (fecx::MODE_R | fecx::MODE_ACTION_MF | fecx::MODE_ACTION_ASM | fecx::MODE_DIAG)) ra.R = R;
   if (mode & fecx::MODE_K) ra.vals = vals;
   const int nblocks = (int)ctx->block_mp.size();
-  const int k = nblocks > 1 ? 1 : pipeline_chunks(ctx, mode);
+  const int k = (nblocks > 1 || noalloc) ? 1 : pipeline_chunks(ctx, mode);
   hipStream_t es = ctx->stream;
   if (k > 1) {
     if (!ctx->stream2) HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
@@ -349,7 +366,11 @@ int fecx_create(fecx_ctx** out, int device) {
   ctx->device = device;
   int rc = guard(ctx, [&] {
     (void)dev(ctx, "flag", sizeof(int));
-    if (device >= 0) HIPCHK(hipHostMalloc((void**)&ctx->flag_host, sizeof(int), hipHostMallocDefault));
+    if (device >= 0) {
+      HIPCHK(hipHostMalloc((void**)&ctx->flag_host, sizeof(int), hipHostMallocDefault));
+      // the allocation-free entry point's per-device scratch address, looked up once per device, device current
+      require(fecx::noalloc_setup(device) == FECX_OK, FECX_ERR_HIP, "scratch symbol lookup failed");
+    }
   });
   if (rc) { delete ctx; return rc; }
   *out = ctx;
@@ -719,6 +740,13 @@ int fecx_assemble_vector_matrix(fecx_ctx* ctx, const double* U, double* R, doubl
   });
 }
 
+int fecx_assemble_vector_matrix_noalloc(fecx_ctx* ctx, const double* U, double* R, double* vals) {
+  return guard(ctx, [&] {
+    require(R != nullptr && (vals != nullptr || ctx->nnz == 0), FECX_ERR_ARG, "R or vals is NULL");
+    assemble(ctx, fecx::MODE_R | fecx::MODE_K, U, nullptr, R, vals, true);
+  });
+}
+
 int fecx_assemble_matrix_action(fecx_ctx* ctx, const double* U, const double* V, double* KV) {
   return guard(ctx, [&] {
     require(V != nullptr && KV != nullptr, FECX_ERR_ARG, "V or KV is NULL");
@@ -811,7 +839,7 @@ int fecx_get_scatter_path(fecx_ctx* ctx, int* path) {
     require(path != nullptr, FECX_ERR_ARG, "path is NULL");
     require(ctx->has_pattern && ctx->has_material, FECX_ERR_STATE, "pattern and material must be set");
     const bool surrogate = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
-    *path = scatter_path(ctx, fecx::MODE_R | fecx::MODE_K, surrogate);
+    *path = scatter_path(ctx, fecx::MODE_R | fecx::MODE_K, surrogate, false);
   });
 }
 

@@ -164,6 +164,14 @@ struct ReduceArgs {
 };
 int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s);
 
+// Allocation-free assembly (noalloc.hip): a file-scope device array of NOALLOC_SCRATCH_BYTES per GPU holds the
+// element-private outputs. noalloc_setup looks up the array's device address on `device` (which must be current) once;
+// noalloc_scratch returns it (nullptr before setup).
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(FECX_NOALLOC_SCRATCH_BYTES);
+constexpr int NOALLOC_MAX_DEVICES = 64;
+int noalloc_setup(int device);
+void* noalloc_scratch(int device);
+
 // DOF plumbing
 void launch_scatter_unknowns(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s);
 void launch_gather_unknowns(int64_t n, const int64_t* idx, const double* src, double* dst, hipStream_t s);
