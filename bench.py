@@ -343,8 +343,9 @@ def main():
     ap.add_argument("--workload", default="neohookean_hex8", choices=sorted(SIZES))
     ap.add_argument("--n", type=int, default=None, help="elements per direction (default: the config's size)")
     ap.add_argument("--chunks", type=int, default=0, help="pipelined pieces per matrix assembly (0 or 1: one piece)")
-    ap.add_argument("--scatter", default="fused", choices=["fused", "two_phase", "atomic", "coloured"],
-                    help="scatter design of the matrix assembly (fused: structured hex8 meshes; two-phase elsewhere)")
+    ap.add_argument("--scatter", default="auto", choices=["fused", "two_phase", "atomic", "coloured", "auto"],
+                    help="scatter design of the matrix assembly (auto: the library's default, fused on structured "
+                         "hex8 meshes with vector-valued materials, two-phase elsewhere)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the full-size parity and CPU legs (profiling runs)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg (J2, tet10 NH)")
@@ -386,7 +387,7 @@ def main():
     Uu = torch.from_numpy(wl.U[asm.dof.unknown_dofs - 1]).to(asm.device)
     fec.set_chunks(asm, args.chunks)
     import ctypes
-    design = ["fused", "two_phase", "atomic", "coloured"].index(args.scatter)
+    design = ["fused", "two_phase", "atomic", "coloured", "auto"].index(args.scatter)
     asm.ctx.call("fecx_set_scatter", ctypes.c_int(design))
     path = ctypes.c_int(-1)
     asm.ctx.call("fecx_get_scatter_path", ctypes.byref(path))

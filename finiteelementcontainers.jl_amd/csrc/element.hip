@@ -70,11 +70,12 @@ constexpr int ELEM_THREADS = 256;  // four wavefronts, 256 / G elements per work
 // Quadrature data of the work-group's elements, and over it, once consumed, their element-matrix blocks: those of one
 // wavefront's elements are contiguous, in element storage order, so the wavefront copies them to HBM as 16-byte
 // vectors (one 1 KB store per instruction) with no per-word element/offset arithmetic.
-template <class LDS, int EPW, int NQ>
+// KS: words between two elements' blocks (LDS::KW, or more where a consumer wants another bank spread).
+template <class LDS, int EPW, int NQ, int KS = LDS::KW>
 struct ElemWork {
   union {
     QPBlock<typename LDS::QP, NQ, qp_pad<typename LDS::QP, NQ>()> qp[EPW];
-    float K[EPW * LDS::KW];
+    float K[EPW * KS];
   };
 };
 
@@ -130,13 +131,43 @@ struct PairJobs {
   }
 };
 
+// One lane's gather inputs: its local node's coordinates (hi/lo FP32 split) and field values. Loaded apart from the
+// element computation so that a caller can issue the loads early (the fused kernel fetches the next layer's while
+// the current one is reduced).
+template <int ND, int NF>
+struct NodeIn {
+  float hi[ND], lo[ND];
+  double u[NF], v[NF];
+};
+template <int ND, int NF>
+__device__ __forceinline__ NodeIn<ND, NF> load_node_in(const ElementArgs& args, int64_t node, bool load, bool with_v) {
+  NodeIn<ND, NF> in;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) { in.hi[d] = 0.0f; in.lo[d] = 0.0f; }
+#pragma unroll
+  for (int i = 0; i < NF; ++i) { in.u[i] = 0.0; in.v[i] = 0.0; }
+  if (load) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) { in.hi[d] = args.xhi[node * ND + d]; in.lo[d] = args.xlo[node * ND + d]; }
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+      in.u[i] = args.U[node * NF + i];
+      if (with_v) in.v[i] = args.V[node * NF + i];
+    }
+  }
+  return in;
+}
+
 // The element computation of one 256-thread work-group, G lanes per element (gather, quadrature points, node phase).
 // Lane group grp = threadIdx.x / G evaluates the element at storage position e (when `valid`). Every element row
 // (residual, action, diagonal) leaves through rel_out(word a * NF + i, value); the element blocks (DO_K) are left in
-// work.K, lane-major per element, when this returns (after a work-group barrier). Used by element_kernel (two-phase
-// assembly) and by fused_hex8_kernel (which reduces the blocks from LDS).
-template <int KIND, int NF, int PHYS, int MODE, class LDS, class WORK, class RelOut>
-__device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e, bool valid, LDS* lds, WORK& work,
+// work.K, lane-major per element, element grp's at word grp * KS (KS = 0: LDS::KW), when this returns (after a
+// work-group barrier). Used by element_kernel (two-phase assembly) and by the structured hex8 kernels (which reduce
+// the blocks from LDS). `in` holds the lane's gather inputs (lanes below NNPE of a valid element; load_node_in), `sel`
+// the element id that indexes the quadrature-point state.
+template <int KIND, int NF, int PHYS, int MODE, int KS = 0, class LDS, class WORK, class RelOut>
+__device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e, int64_t sel, bool valid,
+                                             const NodeIn<ET<KIND>::ND, NF>& in, LDS* lds, WORK& work,
                                              RelOut rel_out, bool& bad, int tid) {
   using T = ET<KIND>;
   using M = Mat<PHYS>;
@@ -145,7 +176,8 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
   constexpr bool DO_R = LDS::DO_R, DO_K = LDS::DO_K, DO_MF = LDS::DO_MF, DO_ASM = LDS::DO_ASM, DO_D = LDS::DO_D;
   constexpr bool DO_G = (MODE & MODE_QPGRAD) != 0;
   constexpr bool NEED_C = LDS::NEED_C, GS = LDS::GS, SC = LDS::SC, QR = LDS::QR, QM = LDS::QM, NOQP = LDS::NOQP;
-  constexpr int KW = LDS::KW;
+  constexpr int KW = KS ? KS : LDS::KW;
+  static_assert(KW >= LDS::KW, "element block stride below the element's words");
 
   const int lane = tid % G, grp = tid / G;
   auto& L = lds[grp];
@@ -179,18 +211,10 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
 
   // ---------------- gather (lane = local node)
   {
-    float hi[ND] = {}, lo[ND] = {};
-    double uu[NF] = {}, vv[NF] = {};
-    if (valid && lane < NNPE) {
-      const int64_t node = args.conn[e * NNPE + lane];
-#pragma unroll
-      for (int d = 0; d < ND; ++d) { hi[d] = args.xhi[node * ND + d]; lo[d] = args.xlo[node * ND + d]; }
-#pragma unroll
-      for (int i = 0; i < NF; ++i) {
-        uu[i] = args.U[node * NF + i];
-        if (DO_MF || DO_ASM) vv[i] = args.V[node * NF + i];
-      }
-    }
+    const float* hi = in.hi;
+    const float* lo = in.lo;
+    const double* uu = in.u;
+    const double* vv = in.v;
     const int src0 = (grp * G) % 64;  // lane (within the wavefront) of local node 0
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
@@ -312,14 +336,14 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
     } else {
       float so[7] = {}, sn[7];
       if constexpr (PHYS == FECX_J2) {
-        const int64_t sidx = ((int64_t)args.eperm[e] * NQ + q) * 7;
+        const int64_t sidx = (sel * NQ + q) * 7;
 #pragma unroll
         for (int st = 0; st < 7; ++st) so[st] = args.st_old[sidx + st];
       }
       M::template eval<ND>(args.mp, H, so, sn, P, C, bad);
       if constexpr (PHYS == FECX_J2) {
         if (DO_R && args.st_new && r == 0) {
-          const int64_t sidx = ((int64_t)args.eperm[e] * NQ + q) * 7;
+          const int64_t sidx = (sel * NQ + q) * 7;
 #pragma unroll
           for (int st = 0; st < 7; ++st) args.st_new[sidx + st] = sn[st];
         }
@@ -611,7 +635,11 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   const int64_t e = args.ebeg + (int64_t)blockIdx.x * EPW + grp;  // element storage position
   const bool valid = e < args.ne;
   bool bad = false;
-  element_eval<KIND, NF, PHYS, MODE>(args, e, valid, lds, work,
+  const bool ld = valid && threadIdx.x % G < NNPE;
+  const auto in = load_node_in<T::ND, NF>(args, ld ? (int64_t)args.conn[e * NNPE + threadIdx.x % G] : 0, ld,
+                                          (MODE & (MODE_ACTION_MF | MODE_ACTION_ASM)) != 0);
+  const int64_t sel = (PHYS == FECX_J2 && valid) ? (int64_t)args.eperm[e] : 0;
+  element_eval<KIND, NF, PHYS, MODE>(args, e, sel, valid, in, lds, work,
                                      [&](int w, float v) { args.Rel[e * NDOF + w] = v; }, bad, threadIdx.x);
   if constexpr (DO_K) {
     // the whole wavefront copies its elements' contiguous storage range with consecutive lanes on consecutive words
@@ -664,162 +692,227 @@ static void launch_modes(int mode, const ElementArgs& a, hipStream_t s) {
 // Jacobian are assembled in one kernel, without the element-private storage of the two-phase path and its round trip
 // through HBM (written once by the element kernel, gathered again by the reduction).
 //
-// A work-group owns FT_X x FT_Y node columns and a range [z0, z1) of node planes, and marches up through the element
-// layers k = z0 - 1 .. z1 - 1. For each layer it evaluates the (FT_X + 1) x (FT_Y + 1) = 32 elements around its columns
-// (element_eval, the same code and bits as the element kernel) and leaves their blocks and rows in LDS. Node plane k
-// touches layers k - 1 and k only, so after layer k:
+// Tiles: TX x TY node columns, each seen through the SX x SY = (TX + 1) x (TY + 1) element columns around it. The work
+// is the list of (tile, node plane) units, tile-major; each work-group (one per slot the device can hold at once) takes
+// one contiguous share of it and marches up through the element layers of each tile range [p0, p1) in its share,
+// k = p0 - 1 .. p1 - 1. For each layer it evaluates the SX x SY elements of the tile (element_eval, the same code and
+// bits as the element kernel) and leaves their blocks and rows in LDS. Node plane k touches layers k - 1 and k only,
+// so after layer k:
 //   - the rows of plane k are complete: each block is its partial sum over layer k - 1, kept in FP64 registers since
 //     the previous layer, plus its blocks from layer k; they are written to the CSR values, each entry once;
-//   - the layer-k blocks of plane k + 1 become the new partial sums (only neighbours one plane up or level with a
-//     node have any: 18 of the 27 blocks of a row).
-// Elements of the halo ring around the columns and of the layer below z0 are evaluated by both work-groups that need
-// them (32 elements per 21 columns). The FP64 sum of an entry runs over its element contributions of layer k - 1 in
-// ascending element id, then over those of layer k: the contributions of the two-phase path, grouped by layer.
+//   - the layer-k blocks of plane k + 1 become the new partial sums (only neighbours one plane down or level with a
+//     node have any).
+// Reduction lanes: lane (dx, dy, node) owns the three blocks (dx, dy, dz = -1, 0, +1) of its node's row: per layer it
+// writes the three of plane k and keeps two partial sums for plane k + 1. A wavefront's lanes share (dx, dy) but for
+// one switch, so the element slots they read are the same up to the node's offset (no per-lane search); slots are
+// WORD-strided by an odd count, so the lanes of one read hit distinct LDS banks. The lanes of (0, 0) also form the
+// residual rows.
+// Elements of the halo ring around a tile and of the layer below p0 are evaluated by both work-groups that need
+// them. The FP64 sum of an entry runs over its element contributions of layer k - 1 in ascending element id, then over
+// those of layer k: the contributions of the two-phase path, grouped by layer.
 // Column slots come from a per-node mask of the 27 neighbours present in its CSR row (existing and free): neighbour
 // offsets (dx, dy, dz) in lexicographic (dz, dy, dx) order are the row's ascending node order.
-#ifndef FUSED_WAVES
-#define FUSED_WAVES 2
-#endif
-constexpr int FT_X = 7, FT_Y = 3, FT_NODES = FT_X * FT_Y;
-constexpr int FT_PBLK = FT_NODES * 18;             // blocks with a partial sum (dz = -1, 0)
-constexpr int FT_BLK = FT_NODES * 27;              // blocks of a plane
-static_assert((FT_X + 1) * (FT_Y + 1) == ELEM_THREADS / 8, "one element layer per work-group pass");
-static_assert(FT_PBLK <= 2 * ELEM_THREADS && FT_BLK <= 3 * ELEM_THREADS, "at most 2 partial and 3 blocks per lane");
-
 __device__ __forceinline__ int hex_local(int ox, int oy, int oz) {  // StructuredMesh.jl:116-123 local order
   return oz * 4 + (oy ? (ox ? 2 : 3) : (ox ? 1 : 0));
 }
 
-template <int NF, int PHYS>
-__global__ __launch_bounds__(ELEM_THREADS) __attribute__((amdgpu_waves_per_eu(FUSED_WAVES, 8)))
+template <int SX, int SY>
+struct FusedTile {
+  static constexpr int EPW = SX * SY, THREADS = 8 * EPW;  // element slots of a layer, 8 lanes each
+  static constexpr int TX = SX - 1, TY = SY - 1, NODES = TX * TY;
+  static constexpr int TASKS = 9 * NODES;  // reduction lanes (dx, dy, node), node fastest
+  static_assert(TASKS <= THREADS, "one reduction task per lane");
+};
+constexpr int FUSED_SX = 8, FUSED_SY = 8;
+
+// LDS of the fused kernel: the element work area, reused between layers as the staging area of the plane's rows
+// (node-major, each node's rows at NB * 27 words in their CSR layout), so that they leave as contiguous stores.
+template <class LDS, int EPW, int KS, int NODES, int NB>
+union FusedWork {
+  ElemWork<LDS, EPW, 8, KS> work;
+  double stage[NODES * 27 * NB];
+};
+
+template <int NF, int PHYS, int SX, int SY>
+__global__ __launch_bounds__(8 * SX * SY) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
+  using TL = FusedTile<SX, SY>;
   using LDS = ElemLDS<FECX_HEX8, NF, PHYS, MODE_R | MODE_K>;
-  constexpr int EPW = ELEM_THREADS / 8, NB = NF * NF, KW = LDS::KW;
+  constexpr int EPW = TL::EPW, NB = NF * NF, RW = 27 * NB;
+  constexpr int KS = LDS::KW | 1;  // odd: the slots read by one instruction fall in distinct banks
+  constexpr int RS = 8 * NF + 1;
   __shared__ LDS lds[EPW];
-  __shared__ __attribute__((aligned(16))) ElemWork<LDS, EPW, 8> work;
-  __shared__ float srow[EPW][8 * NF];  // the layer's element rows (residual)
+  __shared__ __attribute__((aligned(16))) FusedWork<LDS, EPW, KS, TL::NODES, NB> fw;
+  __shared__ float srow[EPW][RS];  // the layer's element rows (residual)
+  __shared__ int2 out_rows[TL::NODES];  // per node of the plane being written: {first CSR block, words} or words 0
+  __shared__ double pmbuf[NB][TL::TASKS];  // per reduction lane: the dz = -1 block of plane k + 1 (layer k only)
+  auto& work = fw.work;
   const int tid = threadIdx.x, grp = tid / 8;
-  const int64_t wg = blockIdx.x;
-  const int tx = (int)(wg % f.tiles_x), ty = (int)((wg / f.tiles_x) % f.tiles_y), tz = (int)(wg / f.tiles_x / f.tiles_y);
-  const int x0 = tx * FT_X, y0 = ty * FT_Y;
-  const int z0 = (int)((int64_t)f.nz * tz / f.tiles_z), z1 = (int)((int64_t)f.nz * (tz + 1) / f.tiles_z);
+  const int sx = grp % SX, sy = grp / SX;  // this lane group's element slot
   const int nex = f.nx - 1, ney = f.ny - 1, nez = f.nz - 1;
-  // this lane's element slot: ex = x0 - 1 + exl, ey = y0 - 1 + eyl
-  const int exl = grp >> 2, eyl = grp & 3;
-  const int ex = x0 - 1 + exl, ey = y0 - 1 + eyl;
-  const bool col_ok = ex >= 0 && ex < nex && ey >= 0 && ey < ney;
-  // this lane's blocks: t = 0 (and t = 1 for tid < FT_PBLK - 256) hold a partial sum; the others (dz = +1) do not
-  int bn[3], bd[3];  // node (li + FT_X lj) and neighbour d of block t, or -1
-#pragma unroll
-  for (int t = 0; t < 3; ++t) {
-    const int b = tid + ELEM_THREADS * t;
-    if (b < FT_PBLK) { bn[t] = b / 18; bd[t] = b % 18; }
-    else if (b < FT_BLK) { bn[t] = (b - FT_PBLK) / 9; bd[t] = 18 + (b - FT_PBLK) % 9; }
-    else { bn[t] = -1; bd[t] = 0; }
-  }
-  double part[2][NB];
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
-#pragma unroll
-    for (int c = 0; c < NB; ++c) part[t][c] = 0.0;
-  double rpart = 0.0;  // residual lanes: tid < FT_NODES * NF, node tid / NF, component tid % NF
+  const int64_t units = (int64_t)f.tiles_x * f.tiles_y * f.nz;
+  int64_t u = units * blockIdx.x / gridDim.x;
+  const int64_t u_end = units * (blockIdx.x + 1) / gridDim.x;
+  // reduction task of this lane
+  const bool task = tid < TL::TASKS;
+  const int node = tid % TL::NODES, dxdy = tid / TL::NODES;
+  const int li = node % TL::TX, lj = node / TL::TX;
+  const int dx = dxdy % 3 - 1, dy = dxdy / 3 - 1;
+  const int dm = (dx + 1) + 3 * (dy + 1);  // neighbour index of (dx, dy, -1); + 9 and + 18 for dz = 0 and + 1
+  const bool res_lane = task && dxdy == 4;
   bool bad = false;
 
-  // sum over the layer's elements holding node (li, lj) at height oz and its neighbour d: the blocks K_ab, in
-  // ascending element id (ex, then ey)
-  auto layer_block = [&](int li, int lj, int d, int oz, double (&acc)[NB]) {
-    const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
-    const int mz = oz + dz;
-    if (mz < 0 || mz > 1) return;
-#pragma unroll 1
-    for (int cc = 0; cc < 4; ++cc) {
-        const int cx = cc >> 1, cy = cc & 1;
-        const int ox = 1 - cx, oy = 1 - cy;  // node's offsets in element slot (li + cx, lj + cy)
-        const int mx = ox + dx, my = oy + dy;
-        const int sx = li + cx, sy = lj + cy, e_x = x0 - 1 + sx, e_y = y0 - 1 + sy;
-        if (mx < 0 || mx > 1 || my < 0 || my > 1 || e_x < 0 || e_x >= nex || e_y < 0 || e_y >= ney) continue;
-        const PairRef pr = pair_ref(8, hex_local(ox, oy, oz), hex_local(mx, my, mz));
-        const float* blk = work.K + (sx * 4 + sy) * KW + pr.slot * NB;
+  // the block K_ab of element slot Ke, a = the node at (ox, oy, oz), b = its neighbour (ox + dx, oy + dy, mz), added
+  // to acc in FP64
+  auto add_block = [&](const float* Ke, int a, int b, double (&acc)[NB]) {
+    const PairRef pr = pair_ref(8, a, b);
+    const float* blk = Ke + pr.slot * NB;
+    float v[NB];
 #pragma unroll
-        for (int i = 0; i < NF; ++i)
+    for (int c = 0; c < NB; ++c) v[c] = blk[c];
 #pragma unroll
-          for (int k = 0; k < NF; ++k) acc[i * NF + k] += (double)blk[pr.ab ? i * NF + k : k * NF + i];
-      }
-  };
-  auto layer_row = [&](int li, int lj, int comp, int oz) {
-    double s = 0.0;
+    for (int i = 0; i < NF; ++i)
 #pragma unroll
-    for (int cx = 0; cx < 2; ++cx)
-#pragma unroll
-      for (int cy = 0; cy < 2; ++cy) {
-        const int sx = li + cx, sy = lj + cy, e_x = x0 - 1 + sx, e_y = y0 - 1 + sy;
-        if (e_x < 0 || e_x >= nex || e_y < 0 || e_y >= ney) continue;
-        s += (double)srow[sx * 4 + sy][hex_local(1 - cx, 1 - cy, oz) * NF + comp];
-      }
-    return s;
+      for (int k = 0; k < NF; ++k) acc[i * NF + k] += (double)(pr.ab ? v[i * NF + k] : v[k * NF + i]);
   };
 
-  for (int k = z0 - 1; k < z1; ++k) {
-    const bool layer_ok = k >= 0 && k < nez;
-    const bool valid = layer_ok && col_ok;
-    const int64_t eid = valid ? ((int64_t)ex * ney + ey) * nez + k : 0;
-    const int64_t e = valid ? (int64_t)f.epos[eid] : 0;
-    // the thread index through an empty asm statement: the element evaluation's per-lane constants (quadrature point,
-    // shape functions) are then recomputed for every layer instead of held in registers across the loop
-    int tid_k = tid;
-    asm volatile("" : "+v"(tid_k));
-    element_eval<FECX_HEX8, NF, PHYS, MODE_R | MODE_K>(args, e, valid, lds, work,
-                                                         [&](int w, float v) { srow[grp][w] = v; }, bad, tid_k);
-    // plane k: complete its rows and write them; plane k + 1: the new partial sums
-    const bool write = k >= z0;
-    // block t of this lane: its plane-k value starts from the partial sum `p` (blocks d < 18) or zero
-    auto finish_block = [&](int t, const double* p) {
-      const int li = bn[t] % FT_X, lj = bn[t] / FT_X, d = bd[t];
-      double acc[NB];
+  while (u < u_end) {
+    const int64_t tile = u / f.nz;
+    const int p0 = (int)(u % f.nz);
+    const int p1 = (int)(u_end - u < (int64_t)(f.nz - p0) ? p0 + (u_end - u) : f.nz);
+    u += p1 - p0;
+    const int x0 = (int)(tile % f.tiles_x) * TL::TX, y0 = (int)(tile / f.tiles_x) * TL::TY;
+    const int ex = x0 - 1 + sx, ey = y0 - 1 + sy;
+    const bool col_ok = ex >= 0 && ex < nex && ey >= 0 && ey < ney;
+    const int gx = x0 + li, gy = y0 + lj;
+    const bool node_ok = task && gx < f.nx && gy < f.ny;
+    // candidate slots around the node: bit 2 cx + cy for slot (li + cx, lj + cy), which holds the node at offsets
+    // (1 - cx, 1 - cy) and the neighbour at (1 - cx + dx, 1 - cy + dy); both inside the element, element in the mesh
+    unsigned cand = 0;
 #pragma unroll
-      for (int c = 0; c < NB; ++c) acc[c] = p ? p[c] : 0.0;
-      if (layer_ok) layer_block(li, lj, d, 0, acc);
-      const int gx = x0 + li, gy = y0 + lj;
-      if (write && gx < f.nx && gy < f.ny && f.vals) {
-        const int64_t n = gx + (int64_t)f.nx * (gy + (int64_t)f.ny * k);
-        const int2 info = f.nodeinfo[n];
-        const uint32_t mask = (uint32_t)info.y;
-        if ((mask >> d) & 1u) {
-          const int deg = __popc(mask), slot = __popc(mask & ((1u << d) - 1u));
-          double* dst = f.vals + (int64_t)NB * info.x + NF * slot;
+    for (int cc = 0; cc < 4; ++cc) {
+      const int cx = cc >> 1, cy = cc & 1, mx = 1 - cx + dx, my = 1 - cy + dy;
+      const int cex = gx - 1 + cx, cey = gy - 1 + cy;
+      if (mx >= 0 && mx <= 1 && my >= 0 && my <= 1 && cex >= 0 && cex < nex && cey >= 0 && cey < ney)
+        cand |= 1u << cc;
+    }
+    double pz[NB], rp[NF];  // plane k + 1 partial sums: block dz = 0, residual (block dz = -1: pmbuf)
 #pragma unroll
-          for (int i = 0; i < NF; ++i)
+    for (int c = 0; c < NB; ++c) pz[c] = 0.0;
 #pragma unroll
-            for (int kk = 0; kk < NF; ++kk)
-              __builtin_nontemporal_store(acc[i * NF + kk], dst + (int64_t)i * NF * deg + kk);
+    for (int c = 0; c < NF; ++c) rp[c] = 0.0;
+    // gather inputs of the lane's local node a = tid % 8 in its slot's element of layer k (node ids and element ids
+    // follow from the structured numbering: no connectivity read)
+    auto fetch = [&](int k) {
+      const int a = tid % 8, oxa = (a == 1 || a == 2 || a == 5 || a == 6) ? 1 : 0, oya = (a & 3) >= 2 ? 1 : 0;
+      const bool ok = col_ok && k >= 0 && k < nez;
+      const int64_t nd = (ex + oxa) + (int64_t)f.nx * ((ey + oya) + (int64_t)f.ny * (k + (a >> 2)));
+      return load_node_in<3, NF>(args, ok ? nd : 0, ok, false);
+    };
+    NodeIn<3, NF> in = fetch(p0 - 1);
+
+    for (int k = p0 - 1; k < p1; ++k) {
+      const bool layer_ok = k >= 0 && k < nez;
+      const bool valid = layer_ok && col_ok;
+      const int64_t eid = valid ? ((int64_t)ex * ney + ey) * nez + k : 0;
+      const bool wr = node_ok && k >= p0;
+      const int64_t n = gx + (int64_t)f.nx * (gy + (int64_t)f.ny * k);
+      const bool wk = wr && f.vals;
+      int2 info = make_int2(0, 0);
+      if (wk) info = f.nodeinfo[n];  // read here, used after the element evaluation
+      // the thread index through an empty asm statement: the element evaluation's per-lane constants (quadrature
+      // point, shape functions) are then recomputed for every layer instead of held in registers across the loop
+      int tid_k = tid;
+      asm volatile("" : "+v"(tid_k));
+      element_eval<FECX_HEX8, NF, PHYS, MODE_R | MODE_K, KS>(args, 0, eid, valid, in, lds, work,
+                                                             [&](int w, float v) { srow[grp][w] = v; }, bad, tid_k);
+      in = fetch(k + 1);  // the next layer's loads, in flight during the reduction
+      double a0[NB], a1[NB], pm[NB];  // plane k: blocks dz = 0, +1; plane k + 1: block dz = -1
+      if (task) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) { a0[c] = pz[c]; a1[c] = 0.0; pm[c] = 0.0; pz[c] = 0.0; }
+        if (layer_ok) {
+#pragma unroll
+          for (int cc = 0; cc < 4; ++cc) {
+            if ((cand >> cc) & 1u) {
+              const int cx = cc >> 1, cy = cc & 1, ox = 1 - cx, oy = 1 - cy, mx = ox + dx, my = oy + dy;
+              const float* Ke = work.K + ((li + cx) + SX * (lj + cy)) * KS;
+              add_block(Ke, hex_local(ox, oy, 0), hex_local(mx, my, 0), a0);
+              add_block(Ke, hex_local(ox, oy, 0), hex_local(mx, my, 1), a1);
+              add_block(Ke, hex_local(ox, oy, 1), hex_local(mx, my, 0), pm);
+              add_block(Ke, hex_local(ox, oy, 1), hex_local(mx, my, 1), pz);
+            }
+          }
+        }
+        if (res_lane) {
+          double r[NF];
+#pragma unroll
+          for (int c = 0; c < NF; ++c) { r[c] = rp[c]; rp[c] = 0.0; }
+          if (layer_ok) {
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+              if ((cand >> cc) & 1u) {
+                const int cx = cc >> 1, cy = cc & 1;
+                const float* row = srow[(li + cx) + SX * (lj + cy)];
+#pragma unroll
+                for (int c = 0; c < NF; ++c) {
+                  r[c] += (double)row[hex_local(1 - cx, 1 - cy, 0) * NF + c];
+                  rp[c] += (double)row[hex_local(1 - cx, 1 - cy, 1) * NF + c];
+                }
+              }
+            }
+          }
+          if (wr && f.R) {
+#pragma unroll
+            for (int c = 0; c < NF; ++c) f.R[n * NF + c] = r[c];
+          }
         }
       }
-    };
-    auto next_partial = [&](int t, double (&p)[NB]) {
+      __syncthreads();  // every lane has read the element blocks: the work area becomes the staging area
+      if (task) {
+        const uint32_t mask = (uint32_t)info.y;
+        const int deg = __popc(mask);
+        double* st = fw.stage + node * RW;
+        auto put = [&](int d, const double (&acc)[NB]) {
+          if ((mask >> d) & 1u) {
+            double* dst = st + NF * __popc(mask & ((1u << d) - 1u));
 #pragma unroll
-      for (int c = 0; c < NB; ++c) p[c] = 0.0;
-      if (layer_ok) layer_block(bn[t] % FT_X, bn[t] / FT_X, bd[t], 1, p);
-    };
-    finish_block(0, part[0]);  // block 0 of every lane has a partial sum (tid < FT_PBLK)
-    next_partial(0, part[0]);
-    if (bn[1] >= 0) {
-      if (bd[1] < 18) {
-        finish_block(1, part[1]);
-        next_partial(1, part[1]);
-      } else {
-        finish_block(1, nullptr);
+            for (int i = 0; i < NF; ++i)
+#pragma unroll
+              for (int kk = 0; kk < NF; ++kk) dst[i * NF * deg + kk] = acc[i * NF + kk];
+          }
+        };
+        if (wk) {
+          double am[NB];  // plane k, dz = -1: layer k - 1 only
+#pragma unroll
+          for (int c = 0; c < NB; ++c) am[c] = pmbuf[c][tid];
+          put(dm, am);
+          put(dm + 9, a0);
+          put(dm + 18, a1);
+        }
+#pragma unroll
+        for (int c = 0; c < NB; ++c) pmbuf[c][tid] = pm[c];
+        if (res_lane) out_rows[node] = make_int2(info.x, wk ? NB * deg : 0);
       }
+      __syncthreads();
+      // the next layer's inputs have arrived before the stores below are issued: the loads of the next element
+      // evaluation then wait for no store
+#pragma unroll
+      for (int d = 0; d < 3; ++d) asm volatile("" ::"v"(in.hi[d]), "v"(in.lo[d]));
+#pragma unroll
+      for (int i = 0; i < NF; ++i) asm volatile("" ::"v"(in.u[i]));
+      // the plane's rows, node by node: each node's NB * deg words are contiguous in the CSR values
+      {
+        const int wave = tid / 64, wl = tid % 64;
+        for (int nd = wave; nd < TL::NODES; nd += TL::THREADS / 64) {
+          const int2 o = out_rows[nd];
+          const double* src = fw.stage + nd * RW;
+          double* dst = f.vals + (int64_t)NB * o.x;
+          for (int j = wl; j < o.y; j += 64) __builtin_nontemporal_store(src[j], dst + j);
+        }
+      }
+      // no barrier here: element_eval's first barrier comes before its first write to the work area
     }
-    if (bn[2] >= 0) finish_block(2, nullptr);
-    if (tid < FT_NODES * NF) {
-      const int nl = tid / NF, comp = tid % NF, li = nl % FT_X, lj = nl / FT_X;
-      double r = rpart;
-      if (layer_ok) r += layer_row(li, lj, comp, 0);
-      const int gx = x0 + li, gy = y0 + lj;
-      if (write && gx < f.nx && gy < f.ny && f.R) f.R[(gx + (int64_t)f.nx * (gy + (int64_t)f.ny * k)) * NF + comp] = r;
-      rpart = layer_ok ? layer_row(li, lj, comp, 1) : 0.0;
-    }
-    __syncthreads();  // the next layer's element_eval overwrites lds, work and srow
   }
   if (bad) atomicOr(args.flag, 1);
 }
@@ -859,7 +952,9 @@ void scatter_hex8_kernel(ElementArgs args, FusedArgs f, int colour) {
   if (!valid) { ex = ey = ez = 0; }
   const int64_t e = valid ? (int64_t)f.epos[((int64_t)ex * ney + ey) * nez + ez] : 0;
   bool bad = false;
-  element_eval<FECX_HEX8, NF, PHYS, MODE_R | MODE_K>(args, e, valid, lds, work,
+  const auto in = load_node_in<3, NF>(args, valid ? (int64_t)args.conn[e * 8 + threadIdx.x % 8] : 0, valid, false);
+  const int64_t sel = valid ? ((int64_t)ex * ney + ey) * nez + ez : 0;
+  element_eval<FECX_HEX8, NF, PHYS, MODE_R | MODE_K>(args, e, sel, valid, in, lds, work,
                                                        [&](int w, float v) { srow[grp][w] = v; }, bad, threadIdx.x);
   if (valid) {
     static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
@@ -924,16 +1019,33 @@ int launch_scatter_hex8(int nf, int scatter, const ElementArgs& a, const FusedAr
   return -1;
 }
 
-int launch_fused_hex8(int nf, const ElementArgs& a, const FusedArgs& f, hipStream_t s) {
-  const unsigned grid = (unsigned)((int64_t)f.tiles_x * f.tiles_y * f.tiles_z);
-  const int phys = a.mp.phys;
-  if (nf == 1 && phys == FECX_POISSON) { fused_hex8_kernel<1, FECX_POISSON><<<grid, ELEM_THREADS, 0, s>>>(a, f); return 0; }
-  if (nf == 3 && phys == FECX_LINEAR_ELASTIC) {
-    fused_hex8_kernel<3, FECX_LINEAR_ELASTIC><<<grid, ELEM_THREADS, 0, s>>>(a, f);
-    return 0;
+// One work-group per slot the device holds at once (the kernel's occupancy times the CU count), each with an equal
+// share of the (tile, plane) units.
+template <int NF, int PHYS>
+static void launch_fused(const ElementArgs& a, FusedArgs f, hipStream_t s) {
+  using TL = FusedTile<FUSED_SX, FUSED_SY>;
+  auto kern = fused_hex8_kernel<NF, PHYS, FUSED_SX, FUSED_SY>;
+  static int slots = 0;  // per instantiation; the same on every gfx950 device
+  if (slots == 0) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+        hipSuccess || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, TL::THREADS, 0) != hipSuccess)
+      cus = per = 0;
+    slots = (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
   }
-  if (nf == 3 && phys == FECX_NEOHOOKEAN) { fused_hex8_kernel<3, FECX_NEOHOOKEAN><<<grid, ELEM_THREADS, 0, s>>>(a, f); return 0; }
-  if (nf == 3 && phys == FECX_J2) { fused_hex8_kernel<3, FECX_J2><<<grid, ELEM_THREADS, 0, s>>>(a, f); return 0; }
+  f.tiles_x = (f.nx + TL::TX - 1) / TL::TX;
+  f.tiles_y = (f.ny + TL::TY - 1) / TL::TY;
+  const int64_t units = (int64_t)f.tiles_x * f.tiles_y * f.nz;
+  const unsigned grid = (unsigned)(units < slots ? units : slots);
+  kern<<<grid, TL::THREADS, 0, s>>>(a, f);
+}
+
+int launch_fused_hex8(int nf, const ElementArgs& a, const FusedArgs& f, hipStream_t s) {
+  const int phys = a.mp.phys;
+  if (nf == 1 && phys == FECX_POISSON) { launch_fused<1, FECX_POISSON>(a, f, s); return 0; }
+  if (nf == 3 && phys == FECX_LINEAR_ELASTIC) { launch_fused<3, FECX_LINEAR_ELASTIC>(a, f, s); return 0; }
+  if (nf == 3 && phys == FECX_NEOHOOKEAN) { launch_fused<3, FECX_NEOHOOKEAN>(a, f, s); return 0; }
+  if (nf == 3 && phys == FECX_J2) { launch_fused<3, FECX_J2>(a, f, s); return 0; }
   return -1;
 }
 

@@ -157,9 +157,11 @@ bool build_nodeinfo(fecx_ctx* ctx) {
 }
 
 // The scatter design a matrix assembly takes (FECX_SCATTER_*): the fused kernel for residual + Jacobian (or Jacobian)
-// assemblies of a one-block structured hex8 mesh with a plain material, unless another design is asked for; the
-// atomic and coloured designs only where they are built (structured hex8, Poisson or neo-Hookean); the two-phase path
-// everywhere else, and always for the allocation-free entry point.
+// assemblies of a one-block structured hex8 mesh with a plain material when asked for, or by default for the
+// vector-valued materials (on the scalar Poisson problem the two-phase path is faster: its element work is cheap and
+// the fused kernel's halo recompute and one-work-group-per-CU marching do not pay); the atomic and coloured designs
+// only where they are built (structured hex8, Poisson or neo-Hookean); the two-phase path everywhere else, and always
+// for the allocation-free entry point.
 int scatter_path(fecx_ctx* ctx, int mode, bool surrogate, bool noalloc) {
   if (!ctx->structured || noalloc || surrogate || ctx->block_mp.size() != 1) return FECX_SCATTER_TWO_PHASE;
   if (mode != (fecx::MODE_R | fecx::MODE_K) && mode != fecx::MODE_K) return FECX_SCATTER_TWO_PHASE;
@@ -168,6 +170,7 @@ int scatter_path(fecx_ctx* ctx, int mode, bool surrogate, bool noalloc) {
                         (ctx->nf == 3 && (ph == FECX_LINEAR_ELASTIC || ph == FECX_NEOHOOKEAN || ph == FECX_J2));
   const bool direct_ok = (ctx->nf == 1 && ph == FECX_POISSON) || (ctx->nf == 3 && ph == FECX_NEOHOOKEAN);
   if (ctx->scatter == FECX_SCATTER_FUSED && fused_ok) return FECX_SCATTER_FUSED;
+  if (ctx->scatter == FECX_SCATTER_AUTO && fused_ok && ctx->nf == 3) return FECX_SCATTER_FUSED;
   if ((ctx->scatter == FECX_SCATTER_ATOMIC || ctx->scatter == FECX_SCATTER_COLOURED) && direct_ok) return ctx->scatter;
   return FECX_SCATTER_TWO_PHASE;
 }
@@ -245,11 +248,6 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   if (path != FECX_SCATTER_TWO_PHASE) {
     fecx::FusedArgs fa{};
     fa.nx = ctx->snx; fa.ny = ctx->sny; fa.nz = ctx->snz;
-    fa.tiles_x = (fa.nx + fecx::FUSED_TILE_X - 1) / fecx::FUSED_TILE_X;
-    fa.tiles_y = (fa.ny + fecx::FUSED_TILE_Y - 1) / fecx::FUSED_TILE_Y;
-    // z ranges of about 16 node planes, at least enough work-groups for three per CU
-    fa.tiles_z = std::max(1, std::min(fa.nz, std::max((fa.nz + 15) / 16, (768 + fa.tiles_x * fa.tiles_y - 1) /
-                                                                           (fa.tiles_x * fa.tiles_y))));
     fa.epos = get<int32_t>(ctx, "epos");
     fa.nodeinfo = get<int2>(ctx, "nodeinfo");
     fa.vals = (mode & fecx::MODE_K) ? vals : nullptr;
@@ -829,7 +827,7 @@ int fecx_extract_field_unknowns(fecx_ctx* ctx, const double* U, double* Uu) {
 
 int fecx_set_scatter(fecx_ctx* ctx, int design) {
   return guard(ctx, [&] {
-    require(design >= FECX_SCATTER_FUSED && design <= FECX_SCATTER_COLOURED, FECX_ERR_ARG, "unknown scatter design");
+    require(design >= FECX_SCATTER_FUSED && design <= FECX_SCATTER_AUTO, FECX_ERR_ARG, "unknown scatter design");
     ctx->scatter = design;
   });
 }

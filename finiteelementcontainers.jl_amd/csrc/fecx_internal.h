@@ -131,13 +131,12 @@ int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t
 // Fused residual + CSR assembly of a structured hex8 mesh (element.hip, fused_hex8_kernel).
 struct FusedArgs {
   int nx, ny, nz;                // node counts per direction
-  int tiles_x, tiles_y, tiles_z;  // work-groups per direction
+  int tiles_x, tiles_y;          // node tiles per direction (set by launch_fused_hex8)
   const int32_t* epos;           // element id -> storage position
   const int2* nodeinfo;          // per node: {first CSR block of its row, mask of the 27 neighbours in the row}
   double* vals;                  // CSR values (or nullptr)
   double* R;                     // full NF x NN residual field (or nullptr)
 };
-constexpr int FUSED_TILE_X = 7, FUSED_TILE_Y = 3;  // node columns per work-group
 // returns 0, or -1 if the (nf, phys) combination has no fused kernel
 int launch_fused_hex8(int nf, const ElementArgs& a, const FusedArgs& f, hipStream_t s);
 // the atomic (scatter 1) and coloured (scatter 2) designs on structured hex8 meshes; vals and R zero-filled first
@@ -213,7 +212,7 @@ struct fecx_ctx {
   // scatter design (fecx_set_scatter)
   int snx = 0, sny = 0, snz = 0;
   bool structured = false;
-  int scatter = FECX_SCATTER_FUSED;  // fecx_set_scatter
+  int scatter = FECX_SCATTER_AUTO;  // fecx_set_scatter
   // pattern
   bool has_pattern = false;
   fecx::PairTable pairs;

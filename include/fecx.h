@@ -342,7 +342,9 @@ int fecx_set_chunks(fecx_ctx* ctx, int chunks);
 
 /* Scatter design of matrix assemblies (no counterpart in the reference, whose element loop stores element-private
  * matrices and sparse! sums them afterwards, src/assemblers/Matrix.jl:35-75, SparsityPatterns.jl:301-329):
- *   FECX_SCATTER_FUSED      (default) on a one-block hex8 mesh numbered as StructuredMesh numbers it
+ *   FECX_SCATTER_AUTO       (default) FUSED where it is the faster design on MI355X (vector-valued materials, nf = 3),
+ *                           TWO_PHASE elsewhere
+ *   FECX_SCATTER_FUSED      on a one-block hex8 mesh numbered as StructuredMesh numbers it
  *                           (src/meshes/StructuredMesh.jl:85-131; detected by fecx_build_pattern), residual + Jacobian
  *                           and Jacobian-only assemblies with a plain material run as one kernel that sums each CSR
  *                           row from the element matrices in on-chip memory; the two-phase path elsewhere
@@ -352,11 +354,12 @@ int fecx_set_chunks(fecx_ctx* ctx, int chunks);
  *   FECX_SCATTER_COLOURED   zero fill, then 8 element colours whose elements share no node, plain adds
  *                           (same meshes as atomic)
  * The designs sum the same FP32 element contributions in FP64 in different orders, so they agree to FP64 rounding.
- * fecx_get_scatter_path reports the design a residual + Jacobian call of the context takes. */
+ * fecx_get_scatter_path reports the design (FUSED .. COLOURED) a residual + Jacobian call of the context takes. */
 #define FECX_SCATTER_FUSED 0
 #define FECX_SCATTER_TWO_PHASE 1
 #define FECX_SCATTER_ATOMIC 2
 #define FECX_SCATTER_COLOURED 3
+#define FECX_SCATTER_AUTO 4
 int fecx_set_scatter(fecx_ctx* ctx, int design);
 int fecx_get_scatter_path(fecx_ctx* ctx, int* path);
 

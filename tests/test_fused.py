@@ -36,7 +36,7 @@ def _brick(fec, counts):
     return fec.BlockMesh(coords, [("block_1", conn)], "HEX8")
 
 
-FUSED, TWO_PHASE, ATOMIC, COLOURED = 0, 1, 2, 3
+FUSED, TWO_PHASE, ATOMIC, COLOURED, AUTO = 0, 1, 2, 3, 4
 
 
 def _applies(fec, ctx):
@@ -105,6 +105,9 @@ def test_fused_matches_two_phase_and_oracle(fec, orc, gpu, name):
         m.nodeset_nodes["_bc"] = np.unique((wl.dirichlet_dofs - 1) // wl.nf + 1)
         bcs = [fec.DirichletBC(nm, lambda X, t: 0.0, nset_name="_bc") for nm in var.names]
     p = fec.create_parameters(m, asm, wl.physics, dirichlet_bcs=bcs)
+    # the default design: fused for the vector-valued materials, two-phase for the scalar Poisson problem
+    assert _applies(fec, asm.ctx) == (1 if wl.nf == 3 else 0)
+    asm.ctx.call("fecx_set_scatter", ctypes.c_int(FUSED))
     assert _applies(fec, asm.ctx) == 1
     U = wl.U.copy()
     U[asm.dof.dirichlet_dofs - 1] = 0.0
