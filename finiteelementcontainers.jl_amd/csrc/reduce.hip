@@ -8,12 +8,13 @@
 // adjacent lanes writing adjacent columns.
 // Vector: the same groups sum the node's incident element rows into the residual field (the atomic scatter of
 // src/assemblers/Assemblers.jl:72-87 restated as an owner-computes gather).
-// Three kernels share this contract:
-//   reduce_rows_kernel          slot-owner lanes (scalar tet10/quad4 rows, 2-field quad4 rows, heavy rows of more than
-//                               72 blocks and heavy hex8/quad4 rows);
-//   reduce_scalar_hex8_kernel   scalar hex8 light rows: lanes add their words into LDS slot accumulators instead;
-//   reduce_scatter_kernel       3-field rows (all light ones, tet10 heavy rows of up to 72 blocks): the same scatter;
-//   reduce_vector_kernel / reduce_vector_group_kernel   calls without CSR values (one thread, or 8 lanes, per row).
+// These kernels share this contract (launch_buckets routes the light rows, then the heavy ones):
+//   reduce_scalar_hex8_kernel   scalar hex8 light rows: lanes add their words into LDS slot accumulators;
+//   reduce_scatter_kernel       3-field light rows of up to 32 blocks, and the tet10 heavy rows (scalar and 3-field)
+//                               of up to 72 blocks: the same scatter, 32 or 64 lanes per row;
+//   reduce_rows_kernel          slot-owner lanes for everything else: light scalar tet10/quad4 rows, 3-field light
+//                               rows of more than 32 blocks, heavy hex8/quad4 rows and tet10 rows of more than 72;
+//   reduce_vector_kernel / reduce_vector_group_kernel   calls without CSR values (one thread, or 16 lanes, per row).
 // All sum a slot's (or a row's) contributions in ascending incidence order, so every form gives the same bits.
 #include <algorithm>
 
@@ -23,24 +24,12 @@ namespace fecx {
 
 constexpr int RED_THREADS = 256;
 constexpr int CH = 8;            // incidences staged per pass
-#ifndef RED_NF1_LPR
-#define RED_NF1_LPR 8            // lanes per scalar row of at most 32 blocks (8 or 16)
-#endif
-#ifndef RED_NF3_LPR
-#define RED_NF3_LPR 32           // lanes per vector row of at most 32 blocks
-#endif
-#ifndef RED_HEAVY_LPR
-#define RED_HEAVY_LPR 128        // lanes per row of 65..256 blocks (one slot per lane and sweep, two sweeps past 128)
-#endif
-#ifndef RED_LB
-#define RED_LB 4                 // vector rows: incidences whose block loads are in flight together
-#endif
-#ifndef RED_LB1
-#define RED_LB1 8                // the same for rows with one staged word per lane and incidence
-#endif
-#ifndef RED_HEAVY_CH
-#define RED_HEAVY_CH 24          // incidences staged per pass on 128-lane rows (tet10 interior vertices: 24)
-#endif
+constexpr int RED_NF1_LPR = 8;     // lanes per scalar row of at most 32 blocks
+constexpr int RED_NF3_LPR = 32;    // lanes per vector row of at most 32 blocks
+constexpr int RED_HEAVY_LPR = 128;  // lanes per row of 65..256 blocks (one slot per lane and sweep, two sweeps past 128)
+constexpr int RED_LB = 4;          // vector rows: incidences whose block loads are in flight together
+constexpr int RED_LB1 = 8;         // the same for rows with one staged word per lane and incidence
+constexpr int RED_HEAVY_CH = 24;   // incidences staged per pass on 128-lane rows (tet10 interior vertices: 24)
 
 // Position of byte == s in the packed slot words (at most one match: a node appears once per element), or -1.
 template <int VW>
@@ -311,10 +300,6 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scalar_hex8_kernel(ReduceA
   for (int k = lane; k < deg; k += LPR) a.vals[(int64_t)ri.z + k] = s_acc[grp][k];
 }
 
-#ifndef RED_SCATTER
-#define RED_SCATTER 1
-#endif
-
 // Vector rows as a scatter, LPR lanes per row within one wavefront (light rows: 32 lanes, at most 8 incidences and 32
 // blocks; tet10 vertex rows: 64 lanes, 24 incidences per pass, 65 blocks). Lane l holds words l, l + LPR, ... of the
 // row's staged K_a* (block b = word / NF^2, entry c = word % NF^2) for each incidence, loaded a batch of incidences
@@ -438,12 +423,6 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scatter_kernel(ReduceArgs 
   }
 }
 
-#ifndef RED_HEAVY_SCATTER
-#define RED_HEAVY_SCATTER 1
-#endif
-#ifndef RED_LIGHT_SCATTER
-#define RED_LIGHT_SCATTER 1
-#endif
 constexpr int HEAVY_SCATTER_SMAX = 72;  // structured tet10 vertex rows: 65 blocks
 
 // One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 8 for
@@ -500,7 +479,7 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
   hi.padded = 0;
   int rc = 0;
   if constexpr (NF == 1 && NNPE == 8) {
-    if (RED_SCATTER && lo.p1 > lo.p0) {
+    if (lo.p1 > lo.p0) {
       if (a.maxdeg_light > 32) return -1;
       int64_t grid = (lo.p1 - lo.p0 + RED_THREADS / 8 - 1) / (RED_THREADS / 8);
       grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
@@ -509,7 +488,7 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
       rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
     }
   } else if constexpr (NF == 3) {
-    if (RED_LIGHT_SCATTER && lo.p1 > lo.p0 && a.maxdeg_light <= 32) {
+    if (lo.p1 > lo.p0 && a.maxdeg_light <= 32) {
       int64_t grid = (lo.p1 - lo.p0 + RED_THREADS / 32 - 1) / (RED_THREADS / 32);
       grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
       reduce_scatter_kernel<NF, NNPE, 32, CH, 32><<<(unsigned)grid, RED_THREADS, 0, s>>>(lo);
@@ -520,7 +499,7 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
     rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
   }
   if (rc != 0 || hi.p1 <= hi.p0) return rc;
-  if constexpr (RED_HEAVY_SCATTER && NNPE == 10) {
+  if constexpr (NNPE == 10) {
     if (a.maxdeg <= HEAVY_SCATTER_SMAX) {
       int64_t grid = (hi.p1 - hi.p0 + RED_THREADS / 64 - 1) / (RED_THREADS / 64);
       grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
@@ -629,15 +608,11 @@ __global__ __launch_bounds__(256) void reduce_vector_group_kernel(ReduceArgs a) 
   if (valid && lane < NF) a.R[(int64_t)ri.x * NF + lane] = acc;
 }
 
-#ifndef RED_VEC_GROUP
-#define RED_VEC_GROUP 1
-#endif
-
 int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s) {
   if (a.p1 <= a.p0) return 0;
   if (!a.vals) {
     if (!a.R) return 0;
-    if (RED_VEC_GROUP && a.nnpe == 10) {
+    if (a.nnpe == 10) {
       // light rows (edge midpoints, at most 8 incidences: one batch of loads) one thread each, then the vertex rows
       // (24 incidences) 16 lanes each, 32 incidences staged per pass (0.370 ms against 0.393 with 8 lanes and 24, and
       // 0.414 with 4, on tet10 80^3 x 6)
