@@ -721,7 +721,8 @@ struct FusedTile {
   static constexpr int EPW = SX * SY, THREADS = 8 * EPW;  // element slots of a layer, 8 lanes each
   static constexpr int TX = SX - 1, TY = SY - 1, NODES = TX * TY;
   static constexpr int TASKS = 9 * NODES;  // reduction lanes (dx, dy, node), node fastest
-  static_assert(TASKS <= THREADS, "one reduction task per lane");
+  static constexpr int RES0 = THREADS - 64;  // residual lanes (node) in the last wavefront
+  static_assert(TASKS <= RES0 && NODES <= 64, "one reduction task per lane, the residual lanes apart");
 };
 constexpr int FUSED_SX = 8, FUSED_SY = 8;
 
@@ -753,27 +754,31 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
   const int64_t units = (int64_t)f.tiles_x * f.tiles_y * f.nz;
   int64_t u = units * blockIdx.x / gridDim.x;
   const int64_t u_end = units * (blockIdx.x + 1) / gridDim.x;
-  // reduction task of this lane
+  // reduction task of this lane: the blocks (dx, dy, *) of a node's row, or the node's residual row (the lanes of the
+  // last wavefront, which also hand the plane's row ranges to the copy-out)
   const bool task = tid < TL::TASKS;
-  const int node = tid % TL::NODES, dxdy = tid / TL::NODES;
+  const bool res_lane = tid >= TL::RES0 && tid < TL::RES0 + TL::NODES;
+  const int node = task ? tid % TL::NODES : (res_lane ? tid - TL::RES0 : 0);
+  const int dxdy = task ? tid / TL::NODES : 4;
   const int li = node % TL::TX, lj = node / TL::TX;
   const int dx = dxdy % 3 - 1, dy = dxdy / 3 - 1;
   const int dm = (dx + 1) + 3 * (dy + 1);  // neighbour index of (dx, dy, -1); + 9 and + 18 for dz = 0 and + 1
-  const bool res_lane = task && dxdy == 4;
   bool bad = false;
 
-  // the block K_ab of element slot Ke, a = the node at (ox, oy, oz), b = its neighbour (ox + dx, oy + dy, mz), added
-  // to acc in FP64
-  auto add_block = [&](const float* Ke, int a, int b, double (&acc)[NB]) {
+  // the block K_ab of element slot Ke, a = the node at (ox, oy, oz), b = its neighbour (ox + dx, oy + dy, mz): its
+  // words (as stored: K_ab, or K_ba when the result is false), then added to acc in FP64
+  auto load_block = [&](const float* Ke, int a, int b, float (&v)[NB]) {
     const PairRef pr = pair_ref(8, a, b);
     const float* blk = Ke + pr.slot * NB;
-    float v[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) v[c] = blk[c];
+    return pr.ab;
+  };
+  auto add_block = [&](const float (&v)[NB], bool ab, double (&acc)[NB]) {
 #pragma unroll
     for (int i = 0; i < NF; ++i)
 #pragma unroll
-      for (int k = 0; k < NF; ++k) acc[i * NF + k] += (double)(pr.ab ? v[i * NF + k] : v[k * NF + i]);
+      for (int k = 0; k < NF; ++k) acc[i * NF + k] += (double)(ab ? v[i * NF + k] : v[k * NF + i]);
   };
 
   while (u < u_end) {
@@ -785,7 +790,7 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
     const int ex = x0 - 1 + sx, ey = y0 - 1 + sy;
     const bool col_ok = ex >= 0 && ex < nex && ey >= 0 && ey < ney;
     const int gx = x0 + li, gy = y0 + lj;
-    const bool node_ok = task && gx < f.nx && gy < f.ny;
+    const bool node_ok = (task || res_lane) && gx < f.nx && gy < f.ny;
     // candidate slots around the node: bit 2 cx + cy for slot (li + cx, lj + cy), which holds the node at offsets
     // (1 - cx, 1 - cy) and the neighbour at (1 - cx + dx, 1 - cy + dy); both inside the element, element in the mesh
     unsigned cand = 0;
@@ -837,14 +842,21 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
             if ((cand >> cc) & 1u) {
               const int cx = cc >> 1, cy = cc & 1, ox = 1 - cx, oy = 1 - cy, mx = ox + dx, my = oy + dy;
               const float* Ke = work.K + ((li + cx) + SX * (lj + cy)) * KS;
-              add_block(Ke, hex_local(ox, oy, 0), hex_local(mx, my, 0), a0);
-              add_block(Ke, hex_local(ox, oy, 0), hex_local(mx, my, 1), a1);
-              add_block(Ke, hex_local(ox, oy, 1), hex_local(mx, my, 0), pm);
-              add_block(Ke, hex_local(ox, oy, 1), hex_local(mx, my, 1), pz);
+              float v0[NB], v1[NB], v2[NB], v3[NB];  // the four blocks' loads in flight together
+              const bool b0 = load_block(Ke, hex_local(ox, oy, 0), hex_local(mx, my, 0), v0);
+              const bool b1 = load_block(Ke, hex_local(ox, oy, 0), hex_local(mx, my, 1), v1);
+              const bool b2 = load_block(Ke, hex_local(ox, oy, 1), hex_local(mx, my, 0), v2);
+              const bool b3 = load_block(Ke, hex_local(ox, oy, 1), hex_local(mx, my, 1), v3);
+              add_block(v0, b0, a0);
+              add_block(v1, b1, a1);
+              add_block(v2, b2, pm);
+              add_block(v3, b3, pz);
             }
           }
         }
-        if (res_lane) {
+      }
+      if (res_lane) {
+        {
           double r[NF];
 #pragma unroll
           for (int c = 0; c < NF; ++c) { r[c] = rp[c]; rp[c] = 0.0; }
@@ -869,7 +881,7 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
         }
       }
       __syncthreads();  // every lane has read the element blocks: the work area becomes the staging area
-      if (task) {
+      if (task || res_lane) {
         const uint32_t mask = (uint32_t)info.y;
         const int deg = __popc(mask);
         double* st = fw.stage + node * RW;
@@ -882,17 +894,20 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
               for (int kk = 0; kk < NF; ++kk) dst[i * NF * deg + kk] = acc[i * NF + kk];
           }
         };
-        if (wk) {
-          double am[NB];  // plane k, dz = -1: layer k - 1 only
+        if (task) {
+          if (wk) {
+            double am[NB];  // plane k, dz = -1: layer k - 1 only
 #pragma unroll
-          for (int c = 0; c < NB; ++c) am[c] = pmbuf[c][tid];
-          put(dm, am);
-          put(dm + 9, a0);
-          put(dm + 18, a1);
+            for (int c = 0; c < NB; ++c) am[c] = pmbuf[c][tid];
+            put(dm, am);
+            put(dm + 9, a0);
+            put(dm + 18, a1);
+          }
+#pragma unroll
+          for (int c = 0; c < NB; ++c) pmbuf[c][tid] = pm[c];
+        } else {
+          out_rows[node] = make_int2(info.x, wk ? NB * deg : 0);
         }
-#pragma unroll
-        for (int c = 0; c < NB; ++c) pmbuf[c][tid] = pm[c];
-        if (res_lane) out_rows[node] = make_int2(info.x, wk ? NB * deg : 0);
       }
       __syncthreads();
       // the next layer's inputs have arrived before the stores below are issued: the loads of the next element
@@ -903,12 +918,22 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
       for (int i = 0; i < NF; ++i) asm volatile("" ::"v"(in.u[i]));
       // the plane's rows, node by node: each node's NB * deg words are contiguous in the CSR values
       {
+        // wavefront w takes nodes w, w + W, ...; each node's rows in chunks of 64 words, every chunk's LDS read issued
+        // before its stores
+        constexpr int W = TL::THREADS / 64, NCH = (RW + 63) / 64, NPW = (TL::NODES + W - 1) / W;
         const int wave = tid / 64, wl = tid % 64;
-        for (int nd = wave; nd < TL::NODES; nd += TL::THREADS / 64) {
-          const int2 o = out_rows[nd];
+#pragma unroll 1
+        for (int t = 0; t < NPW; ++t) {
+          const int nd = wave + W * t;
+          const int2 o = nd < TL::NODES ? out_rows[nd] : make_int2(0, 0);
           const double* src = fw.stage + nd * RW;
+          double v[NCH];
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) v[c] = wl + 64 * c < o.y ? src[wl + 64 * c] : 0.0;
           double* dst = f.vals + (int64_t)NB * o.x;
-          for (int j = wl; j < o.y; j += 64) __builtin_nontemporal_store(src[j], dst + j);
+#pragma unroll
+          for (int c = 0; c < NCH; ++c)
+            if (wl + 64 * c < o.y) __builtin_nontemporal_store(v[c], dst + wl + 64 * c);
         }
       }
       // no barrier here: element_eval's first barrier comes before its first write to the work area
