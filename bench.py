@@ -108,7 +108,72 @@ MLP_FLOPS = {"j2_hex8": 2 * (13 * 128 + 128 * 128 * 2 + 128 * 13), "neohookean_t
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def surrogate_leg(fec, torch, name, wl, asm_plain, R_plain, K_plain, steps):
+def newton_leg(fec, torch, net, n, steps, tol=1e-4, log_every=True):
+    """The J2 workload as a quasi-static solve: `steps` load steps of the seeded macroscopic strain path on n^3 hex8,
+    each by Newton + Jacobi-CG to a relative residual of `tol`, once with the plain radial return and once with the
+    trained surrogate under SurrogateSchedule (decide, freeze, hand over to the plain model when the surrogate
+    iterations stall). Reports Newton and CG iterations, wall time, the solutions' difference, the plastic fraction of
+    the final state, and the plain residual + Jacobian assembly timed at that plastic state."""
+    params, scaling = net.params(), net.scaling()
+    # tol: above the floor that FP32 element arithmetic puts under the relative residual (about 2e-5 at 64^3: interior
+    # residuals are cancelling sums of O(1) element contributions)
+    st = fec.NewtonSolverSettings(max_iters=15, rel_residual_tol=tol, abs_residual_tol=0.0, abs_increment_tol=0.0)
+    res = {}
+    for surr in (False, True):
+        holder = {}
+
+        def setup(asm):
+            if surr:
+                nt = fec.Surrogate(asm, fec.surrogate.MLP_J2, seed=0)
+                nt.set_params(params)
+                nt.set_scaling(*scaling)
+                nt.set_tolerance(0.05)
+                holder["s"] = fec.solvers.SurrogateSchedule(nt)
+                holder["phases"] = []
+
+        def schedule(it, h):
+            if "s" in holder:
+                holder["s"](it, h)
+
+        def log_step(k, hist):
+            if "s" in holder:
+                holder["phases"].append(list(holder["s"].phase))
+            if log_every:
+                log(f"[bench] newton {'surrogate' if surr else 'plain'} step {k}: {len(hist)} iterations, "
+                    f"rel {hist[-1].rel_R:.1e}")
+
+        torch.cuda.synchronize()
+        t0 = time.time()
+        asm, p, Uu, hists, _ = fec.solvers.j2_load_steps(fec, n, steps=steps, newton_settings=st, setup=setup,
+                                                          surrogate=surr, schedule=schedule, log=log_step)
+        torch.cuda.synchronize()
+        wall = time.time() - t0
+        r = {"newton_iterations": int(sum(len(h) for h in hists)),
+             "per_step": [len(h) for h in hists],
+             "cg_iterations": int(sum(rec.cg_iters for h in hists for rec in h)),
+             "converged_all": bool(all(h[-1].converged for h in hists)),
+             "max_final_rel_residual": float(max(h[-1].rel_R for h in hists)),
+             "wall_s": wall}
+        if surr:
+            ph = holder["phases"]
+            r["surrogate_iterations"] = int(sum(x.count("surrogate") for x in ph))
+            r["plain_iterations_after_handover"] = int(sum(x.count("plain") for x in ph))
+        else:
+            ne = n ** 3
+            state = np.zeros(ne * 8 * 7, dtype=np.float32)
+            import ctypes
+            asm.ctx.call("fecx_get_state_new", state.ctypes.data_as(ctypes.c_void_p))
+            r["plastic_point_fraction_final"] = float((state.reshape(-1, 7)[:, 6] > 0).mean())
+            r["assembly_ms_at_final_state"] = cuda_ms(torch, lambda: fec.assemble_vector_and_stiffness_(asm, Uu, p))
+        res["surrogate" if surr else "plain"] = r
+        res["_U_" + ("s" if surr else "p")] = Uu.cpu().numpy().copy()
+    a, b = res.pop("_U_s"), res.pop("_U_p")
+    res["solution_rel_diff"] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    res["workload"] = f"j2_hex8 {n}^3, {steps} load steps, Newton to relative residual {tol:g}, Jacobi-CG"
+    return res
+
+
+def surrogate_leg(fec, torch, name, wl, asm_plain, R_plain, K_plain, steps, newton_n=0):
     """The learned path of the J2 and tet10 neo-Hookean configurations: train the network on the GPU on plain-model
     samples (J2: 2M samples of random macroscopic strain paths; NH: 2M quadrature points of this workload), then
     measure the held-out acceptance and error at the residual-check tolerance 0.05, the batched material update of
@@ -175,6 +240,16 @@ def surrogate_leg(fec, torch, name, wl, asm_plain, R_plain, K_plain, steps):
         "assembly_fallback_fraction": fb_a / max(np_a, 1),
         "assembly_rel_diff_vs_plain": {"residual": rel(R, R_plain), "jacobian": rel(K, K_plain)},
     }
+    # what the learned path could reach at best: the network's flops at the BF16 MFMA spec and at the measured GEMM
+    # peak, against the plain batched update (the network also moves the same bytes, so this is an upper bound)
+    peak_ms = MLP_FLOPS[name] * nb / (MFMA_BF16_PEAK_TFLOPS * 1e12) * 1e3
+    gemm_ms = MLP_FLOPS[name] * nb / (out["mfma"]["measured_gemm_peak"] * 1e12) * 1e3
+    out["bound"] = {"network_flops_per_point": MLP_FLOPS[name], "network_ms_2M_at_spec_peak": peak_ms,
+                    "network_ms_2M_at_measured_gemm_peak": gemm_ms, "plain_ms_2M": t_plain,
+                    "max_ratio_vs_plain_at_spec_peak": t_plain / peak_ms,
+                    "max_ratio_vs_plain_at_measured_gemm_peak": t_plain / gemm_ms}
+    if j2 and newton_n:
+        out["newton"] = newton_leg(fec, torch, net, newton_n, 20)
     return out
 
 
@@ -350,6 +425,9 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the full-size parity and CPU legs (profiling runs)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg (J2, tet10 NH)")
     ap.add_argument("--no-operators", action="store_true", help="skip the timings of the other entry points")
+    ap.add_argument("--newton", type=int, default=0, metavar="N",
+                    help="j2_hex8 learned leg: also solve the 20 load steps on N^3 elements by Newton, plain and "
+                         "with the surrogate (N = 64: the configuration's full size)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (HBM counter calibration)")
     ap.add_argument("--hbm-ceiling", action="store_true",
@@ -468,7 +546,7 @@ def main():
         fec.assemble_vector_and_stiffness_(asm, Uu, p)
         R_plain = fec.residual(asm).cpu().numpy().copy()
         K_plain = asm.stiffness_storage.cpu().numpy().copy()
-        out["surrogate"] = surrogate_leg(fec, torch, name, wl, asm, R_plain, K_plain, args.steps)
+        out["surrogate"] = surrogate_leg(fec, torch, name, wl, asm, R_plain, K_plain, args.steps, args.newton)
         out["surrogate"]["assembly_ms_per_step"]["plain_step_time_above"] = dt * 1e3
 
     if not args.no_check:
