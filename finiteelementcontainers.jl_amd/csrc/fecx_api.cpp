@@ -175,6 +175,14 @@ int scatter_path(fecx_ctx* ctx, int mode, bool surrogate, bool noalloc) {
   return FECX_SCATTER_TWO_PHASE;
 }
 
+// With a partial Dirichlet set: the dof-level CSR values out of the node-block values the kernels assembled
+// (stream-ordered after them).
+void compact_values(fecx_ctx* ctx, const double* full, double* out) {
+  if (!ctx->partial || !out || ctx->nnz == 0) return;
+  fecx::launch_gather_unknowns(ctx->nnz, get<int64_t>(ctx, "cmap"), full, out, ctx->stream);
+  HIPCHK(hipGetLastError());
+}
+
 // The numeric flag before a launch: cleared on the device only if the last read-back did not find it clear.
 int* arm_flag(fecx_ctx* ctx) {
   int* flag = get<int>(ctx, "flag");
@@ -206,6 +214,8 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
               bool noalloc = false) {
   check_ready(ctx);
   require(U != nullptr, FECX_ERR_ARG, "U is NULL");
+  double* const vals_out = vals;  // the caller's CSR; with a partial Dirichlet set the kernels fill vals_full
+  if (ctx->partial && vals) vals = get<double>(ctx, "vals_full");
   const bool surrogate_phys = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
   const bool surrogate = surrogate_phys && ctx->sg_enabled;
   float *kel = get<float>(ctx, "Kel"), *rel = get<float>(ctx, "Rel");
@@ -258,13 +268,14 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
     } else {  // atomic or coloured: zero-fill, then every element adds its blocks and rows in place
       const size_t rbytes = sizeof(double) * (size_t)ctx->nn * ctx->nf;
       prof.run(path == FECX_SCATTER_ATOMIC ? "atomic" : "coloured", ctx->stream, [&] {
-        if (fa.vals) HIPCHK(hipMemsetAsync(fa.vals, 0, sizeof(double) * (size_t)ctx->nnz, ctx->stream));
+        if (fa.vals) HIPCHK(hipMemsetAsync(fa.vals, 0, sizeof(double) * (size_t)ctx->nnz_full, ctx->stream));
         if (fa.R) HIPCHK(hipMemsetAsync(fa.R, 0, rbytes, ctx->stream));
         rc = fecx::launch_scatter_hex8(ctx->nf, path == FECX_SCATTER_ATOMIC ? 1 : 2, ea, fa, ctx->stream);
       });
     }
     require(rc == 0, FECX_ERR_UNSUPPORTED, "no kernel of this scatter design for this nf / physics");
     HIPCHK(hipGetLastError());
+    compact_values(ctx, vals, vals_out);
     const int hflag = read_flag(ctx, flag);
     prof.finish();
     require(hflag == 0, FECX_ERR_NUMERIC, "element kernel met a non-positive Jacobian determinant or a NaN");
@@ -317,6 +328,7 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
     require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
     HIPCHK(hipGetLastError());
   }
+  compact_values(ctx, vals, vals_out);
   const int hflag = read_flag(ctx, flag);
   prof.finish();
   if (surrogate) fecx::surrogate_qp_finish(ctx);
@@ -339,10 +351,11 @@ void assemble_mass(fecx_ctx* ctx, int what, double density, const double* V, dou
   HIPCHK(hipGetLastError());
   fecx::ReduceArgs ra = reduce_args(ctx);
   ra.R = R;
-  ra.vals = vals;
+  ra.vals = (ctx->partial && vals) ? get<double>(ctx, "vals_full") : vals;
   prof.run("reduce", ctx->stream, [&] { rc = fecx::launch_reduce(ctx->nf, ra, ctx->stream); });
   require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
   HIPCHK(hipGetLastError());
+  compact_values(ctx, ra.vals, vals);
   const int hflag = read_flag(ctx, flag);
   prof.finish();
   require(hflag == 0, FECX_ERR_NUMERIC, "mass kernel met a non-positive Jacobian determinant");
@@ -454,13 +467,14 @@ int fecx_set_dofs(fecx_ctx* ctx, int nf, const int64_t* dirichlet, int64_t ndir)
     for (int64_t d = 1; d <= N; ++d)
       if (!isbc[d]) ctx->unknown_dofs.push_back(d);
     ctx->node_free.assign(ctx->nn, 1);
-    // condensed mode keeps every dof in the pattern, so any Dirichlet set is accepted there
+    ctx->partial = false;
+    // condensed mode keeps every dof in the pattern; otherwise a node stays in the node-block pattern while any of its
+    // dofs is free, and a node with some dofs constrained makes the dof-level compaction necessary
     for (int64_t n = 0; n < ctx->nn && !ctx->condensed; ++n) {
       int c = 0;
       for (int i = 0; i < nf; ++i) c += isbc[n * nf + i + 1];
-      require(c == 0 || c == nf, FECX_ERR_UNSUPPORTED,
-              "each node's dofs must be all free or all Dirichlet when nf > 1 (unless in condensed mode)");
-      ctx->node_free[n] = c == 0;
+      ctx->node_free[n] = c < nf;
+      if (c > 0 && c < nf) ctx->partial = true;
     }
     std::vector<int64_t> u0(ctx->unknown_dofs.size()), d0(ndir);
     for (size_t i = 0; i < u0.size(); ++i) u0[i] = ctx->unknown_dofs[i] - 1;
@@ -500,7 +514,45 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
     const auto& hp = ctx->hp;
     // column slots are bytes with 0xFF for "not a free node"; the reduction takes rows of up to 256 blocks
     require(hp.maxdeg <= 255, FECX_ERR_UNSUPPORTED, "a node couples to more than 255 nodes");
-    ctx->nnz = hp.nnzb * nf * nf;
+    ctx->nnz_full = ctx->nnz = hp.nnzb * nf * nf;
+    ctx->rptr.clear();
+    ctx->rcol.clear();
+    ctx->red_of_full.clear();
+    if (ctx->partial) {
+      // the reference's reduced CSR over the unknown dofs (_update_dofs!, src/DofManagers.jl; SparsityPatterns.jl):
+      // rows and columns in ascending dof order, which is node-block order restricted to the unknown dofs
+      std::vector<int64_t> ridx((size_t)ctx->nn * nf, -1);
+      for (size_t i = 0; i < ctx->unknown_dofs.size(); ++i) ridx[ctx->unknown_dofs[i] - 1] = (int64_t)i;
+      std::vector<int64_t> cmap;
+      ctx->red_of_full.assign((size_t)ctx->nnz_full, -1);
+      ctx->rptr.push_back(0);
+      for (int64_t r = 0; r < hp.nfree; ++r) {
+        const int64_t n = hp.free_node[r], b0 = hp.bptr[r], deg = hp.bptr[r + 1] - b0;
+        for (int i = 0; i < nf; ++i) {
+          if (ridx[n * nf + i] < 0) continue;
+          require(ridx[n * nf + i] == (int64_t)ctx->rptr.size() - 1, FECX_ERR_STATE, "unknown dof order");
+          for (int64_t sl = 0; sl < deg; ++sl) {
+            const int64_t m = hp.free_node[hp.bcol[b0 + sl]];
+            for (int k = 0; k < nf; ++k) {
+              if (ridx[m * nf + k] < 0) continue;
+              const int64_t full = (int64_t)nf * nf * b0 + (int64_t)i * nf * deg + (int64_t)nf * sl + k;
+              ctx->red_of_full[full] = (int64_t)cmap.size();
+              cmap.push_back(full);
+              ctx->rcol.push_back(ridx[m * nf + k]);
+            }
+          }
+          ctx->rptr.push_back((int64_t)ctx->rcol.size());
+        }
+      }
+      require((int64_t)ctx->rptr.size() == (int64_t)ctx->unknown_dofs.size() + 1, FECX_ERR_STATE, "row count");
+      ctx->nnz = (int64_t)cmap.size();
+      require(ctx->nnz < (int64_t(1) << 31), FECX_ERR_UNSUPPORTED, "more than 2^31 - 1 CSR entries");
+      std::vector<int32_t> rcol32(ctx->rcol.begin(), ctx->rcol.end());
+      upload(ctx, "cmap", cmap);
+      upload(ctx, "rptr", ctx->rptr);
+      upload(ctx, "rcol", rcol32);
+      dev(ctx, "vals_full", sizeof(double) * (size_t)std::max<int64_t>(ctx->nnz_full, 1));
+    }
     std::vector<int32_t> conn_s(ctx->ne * ctx->nnpe);  // connectivity in element storage order
     for (int64_t s = 0; s < ctx->ne; ++s)
       for (int a = 0; a < ctx->nnpe; ++a) conn_s[s * ctx->nnpe + a] = ctx->conn0[(int64_t)hp.eperm[s] * ctx->nnpe + a];
@@ -578,6 +630,13 @@ int fecx_get_pattern(fecx_ctx* ctx, int64_t* rowptr, int64_t* colval, int base) 
     require(base == 0 || base == 1, FECX_ERR_ARG, "index base must be 0 or 1");
     const auto& hp = ctx->hp;
     const int nf = ctx->nf;
+    if (ctx->partial) {
+      for (size_t i = 0; i < ctx->rptr.size(); ++i)
+        if (rowptr) rowptr[i] = ctx->rptr[i] + base;
+      for (size_t i = 0; i < ctx->rcol.size(); ++i)
+        if (colval) colval[i] = ctx->rcol[i] + base;
+      return;
+    }
     int64_t pos = 0;
     for (int64_t r = 0; r < hp.nfree; ++r) {
       const int64_t deg = hp.bptr[r + 1] - hp.bptr[r];
@@ -617,7 +676,11 @@ int fecx_get_coo_map(fecx_ctx* ctx, int64_t* map) {
               deg = hp.bptr[ra + 1] - hp.bptr[ra];
               base = (int64_t)nf * nf * hp.bptr[ra] + (int64_t)i * nf * deg + (int64_t)nf * s;
             }
-            for (int k = 0; k < nf; ++k) row[b * nf + k] = base < 0 ? -1 : base + k;
+            for (int k = 0; k < nf; ++k) {
+              int64_t v = base < 0 ? -1 : base + k;
+              if (v >= 0 && ctx->partial) v = ctx->red_of_full[v];  // -1 where the row or the column is constrained
+              row[b * nf + k] = v;
+            }
           }
         }
       }

@@ -217,7 +217,13 @@ struct fecx_ctx {
   bool has_pattern = false;
   fecx::PairTable pairs;
   fecx::HostPattern hp;
-  int64_t nnz = 0;
+  int64_t nnz = 0;       // entries of the CSR the caller sees
+  int64_t nnz_full = 0;  // entries of the node-block layout the kernels assemble (NF^2 per block)
+  // a Dirichlet set that constrains some but not all dofs of a node (nf > 1, not condensed): the kernels assemble the
+  // node-block pattern of the nodes with a free dof into "vals_full", and a gather through "cmap" forms the reference's
+  // dof-level CSR (rows and columns of the unknown dofs); rptr/rcol are that CSR (0-based), red_of_full its inverse map
+  bool partial = false;
+  std::vector<int64_t> rptr, rcol, red_of_full;
   // material: one per element block (foreach_block's per-block physics and properties); mp is block 0's
   bool has_material = false;
   fecx::MatParams mp{};

@@ -17,6 +17,8 @@ struct CsrView {
   const int64_t* bptr;      // free nodes + 1
   const int32_t* bcol;      // nnzb
   const double* vals;       // nnz
+  const int64_t* rptr;      // dof-level CSR (a Dirichlet set that splits nodes): nrows + 1, or null
+  const int32_t* rcol;      // its column indices
 };
 
 // work-group sum of one double per thread (fixed tree), written by thread 0
@@ -39,7 +41,9 @@ __global__ __launch_bounds__(SV_THREADS) void spmv_kernel(CsrView A, const doubl
   const int lane = threadIdx.x % SV_LANES;
   const int64_t row = (int64_t)blockIdx.x * (SV_THREADS / SV_LANES) + threadIdx.x / SV_LANES;
   double acc = 0.0;
-  if (row < A.nrows) {
+  if (row < A.nrows && A.rptr) {
+    for (int64_t j = A.rptr[row] + lane; j < A.rptr[row + 1]; j += SV_LANES) acc += A.vals[j] * x[A.rcol[j]];
+  } else if (row < A.nrows) {
     const int64_t r = row / NF;
     const int i = (int)(row - r * NF);
     const int64_t rb = A.bptr[r];
@@ -66,6 +70,17 @@ template <int NF>
 __global__ void diag_inv_kernel(CsrView A, double* dinv) {
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= A.nrows) return;
+  if (A.rptr) {  // dof-level CSR: binary search for the diagonal column
+    int64_t lo = A.rptr[row], hi = A.rptr[row + 1] - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (A.rcol[mid] < row) lo = mid + 1;
+      else hi = mid;
+    }
+    const double d = (lo < A.rptr[row + 1] && A.rcol[lo] == row) ? A.vals[lo] : 0.0;
+    dinv[row] = d != 0.0 ? 1.0 / d : 1.0;
+    return;
+  }
   const int64_t r = row / NF;
   const int i = (int)(row - r * NF);
   const int64_t rb = A.bptr[r];
@@ -244,6 +259,11 @@ fecx::CsrView csr_view(fecx_ctx* ctx, const double* vals) {
   A.bptr = get<int64_t>(ctx, "bptr");
   A.bcol = get<int32_t>(ctx, "bcol");
   A.vals = vals;
+  if (ctx->partial) {
+    A.nrows = (int64_t)ctx->unknown_dofs.size();
+    A.rptr = get<int64_t>(ctx, "rptr");
+    A.rcol = get<int32_t>(ctx, "rcol");
+  }
   return A;
 }
 

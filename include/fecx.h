@@ -105,8 +105,10 @@ int fecx_get_num_blocks(fecx_ctx* ctx, int* nblocks);
 
 /* nf fields per node and the Dirichlet DOF list, sorted and unique, 1-based (the reference's
  * dirichlet_dofs(bcs), src/bcs/DirichletBCs.jl:396-398). Performs update_dofs!
- * (src/DofManagers.jl:227-298) without periodic pairs. Outside condensed mode, for nf > 1 each node's
- * dofs must be all free or all constrained (FECX_ERR_UNSUPPORTED otherwise). */
+ * (src/DofManagers.jl:227-298) without periodic pairs. For nf > 1 the list may fix single components of a node (a
+ * DirichletBC on one component, src/bcs/DirichletBCs.jl:49-98): the CSR of fecx_get_pattern is then the reference's
+ * dof-level pattern over the unknown dofs; the kernels assemble the node blocks of every node with a free dof and one
+ * gather forms it (the solver calls take that CSR too). */
 int fecx_set_dofs(fecx_ctx* ctx, int nf, const int64_t* dirichlet_dofs, int64_t n_dirichlet);
 
 /* Condensed mode, SparseMatrixAssembler(var; use_condensed = true) (src/assemblers/SparseMatrixAssembler.jl:126-133,

@@ -49,13 +49,6 @@ def test_pattern_and_map_bit_exact(fec, orc, name):
     assert np.array_equal(hp.coo_map(), full)
 
 
-def test_partial_node_dirichlet_rejected(fec):
-    mesh = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (3, 3, 3))
-    with pytest.raises(fec.FECXError) as ei:
-        fec.HostPattern(mesh, 3, np.array([1], np.int64))  # only the x dof of node 1
-    assert ei.value.code == -4
-
-
 def test_bad_inputs_raise(fec):
     mesh = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (3, 3, 3))
     with pytest.raises(fec.FECXError) as ei:
