@@ -135,8 +135,13 @@ def test_host_condensed_pattern_matches_oracle(fec, orc):
     rpo, cvo, _, _ = orc.sparse_csr(N, Is, Js)
     assert np.array_equal(rp, rpo) and np.array_equal(cv, cvo)
     assert np.array_equal(hp.unknown_dofs, np.setdiff1d(np.arange(1, N + 1), wl.dirichlet_dofs))
-    with pytest.raises(fec.FECXError):
-        fec.HostPattern(wl.mesh, 3, wl.dirichlet_dofs, condensed=False)  # reduced mode needs whole nodes
+    # the same per-component set without condensed mode: the reference's dof-level reduced pattern
+    hr = fec.HostPattern(wl.mesh, 3, wl.dirichlet_dofs, condensed=False)
+    unknown, d2u = orc.update_dofs(N, wl.dirichlet_dofs)
+    Is, Js, _, _ = orc.pattern(3, wl.mesh.conn, d2u)
+    rpo, cvo, _, _ = orc.sparse_csr(len(unknown), Is, Js)
+    rp, cv = hr.pattern(1)
+    assert np.array_equal(rp, rpo) and np.array_equal(cv, cvo)
 
 
 # ------------------------------------------------------------------ GPU parity
