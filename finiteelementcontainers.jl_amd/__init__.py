@@ -8,7 +8,7 @@ from .mesh import (StructuredMesh, TetMesh, BlockMesh, split_into_blocks, fan_te
                    boundary_nodes)
 from .exodus import ExodusMesh, UnstructuredMesh, write_exodus  # noqa: F401
 from .assembler import (  # noqa: F401
-    Poisson, Mechanics, NeoHookean, J2Plasticity, ThreeDimensional,
+    Poisson, Mechanics, NeoHookean, J2Plasticity, ThreeDimensional, PlaneStrain,
     residual, stiffness, stiffness_action, mass, lumped_mass, diagonal, density,
     FunctionSpace, ScalarFunction, VectorFunction, DofManager, DirichletBC, DirichletBCs,
     CSRMatrix, CSCMatrix, SparseMatrixAssembler, HostPattern, Parameters, create_parameters, create_unknowns, create_field,

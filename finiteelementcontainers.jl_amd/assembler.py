@@ -53,16 +53,25 @@ class Poisson(Physics):
 
 class ThreeDimensional:
     """Formulation tag (src/Formulations.jl:456-611)."""
+    num_dims = 3
+
+
+class PlaneStrain:
+    """Formulation tag (src/Formulations.jl:318-427): 2D displacement gradients embedded in 3D with a zero third row
+    and column; stress and tangent restricted to the in-plane components. Mechanics only (the test linear-elastic
+    model on quad4 meshes with two displacement fields)."""
+    num_dims = 2
 
 
 class Mechanics(Physics):
     """The reference's test Mechanics physics: psi = K/2 tr(eps)^2 + G dev eps : dev eps
-    (test/mechanics/TestMechanicsCommon.jl:1-236); props = (rho, K, G)."""
+    (test/mechanics/TestMechanicsCommon.jl:1-236); props = (rho, K, G). Mechanics(PlaneStrain()) is its 2D form
+    (test/mechanics/TestMechanics.jl:24)."""
     kind = _lib.LINEAR_ELASTIC
-    nf = 3
 
     def __init__(self, formulation=None, rho=1e3, K=10.0e9, G=1.0e9):
         self.formulation = formulation or ThreeDimensional()
+        self.nf = self.formulation.num_dims
         self.p = [rho, K, G]
 
     def props(self):

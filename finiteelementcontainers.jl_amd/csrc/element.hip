@@ -1082,6 +1082,10 @@ int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t
     if (kind == FECX_TET10) { launch_modes<FECX_TET10, 1, FECX_POISSON>(mode, a, s); return 0; }
     return -1;
   }
+  if (nf == 2 && kind == FECX_QUAD4 && phys == FECX_LINEAR_ELASTIC) {  // plane strain
+    launch_modes<FECX_QUAD4, 2, FECX_LINEAR_ELASTIC>(mode, a, s);
+    return 0;
+  }
   if (nf == 3 && (kind == FECX_HEX8 || kind == FECX_TET10)) {
     if (kind == FECX_HEX8) {
       if (phys == FECX_LINEAR_ELASTIC) { launch_modes<FECX_HEX8, 3, FECX_LINEAR_ELASTIC>(mode, a, s); return 0; }

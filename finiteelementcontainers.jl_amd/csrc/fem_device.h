@@ -138,48 +138,58 @@ struct Mat<FECX_POISSON> {
 // (test/mechanics/TestMechanicsCommon.jl:14-20): A = (K - 2G/3) I(x)I + G (d_ik d_jl + d_il d_jk).
 template <>
 struct Mat<FECX_LINEAR_ELASTIC> {
+  // ND = 3, or ND = 2 in plane strain: the 2 x 2 gradient embedded in a 3 x 3 one with a zero third row and column
+  // (modify_field_gradients, src/Formulations.jl:421-427), whose stress and tangent restricted to the in-plane
+  // components (extract_stress / extract_stiffness, Formulations.jl:391-419) are the same formulas over ND indices
   static constexpr bool FROM_QP = false;  // evaluated in the element kernel
   static constexpr int NF = 3, NC = 1, NV = 3, NS = 0, S0 = 0;
   template <int ND>
   __device__ static void eval(const MatParams& mp, const float* H, const float*, float*, float* P, float*, bool&) {
     const float K = mp.p[1], G = mp.p[2], lm = K - 2.0f * G / 3.0f;
-    const float tr = H[0] + H[4] + H[8];
+    float tr = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < ND; ++i) tr += H[i * ND + i];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) P[i * 3 + j] = G * (H[i * 3 + j] + H[j * 3 + i]) + (i == j ? lm * tr : 0.0f);
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) P[i * ND + j] = G * (H[i * ND + j] + H[j * ND + i]) + (i == j ? lm * tr : 0.0f);
   }
   template <int ND>
   __device__ static void prep(const MatParams&, const float*, const float* g, float* v) {
-    v[0] = g[0]; v[1] = g[1]; v[2] = g[2];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) v[j] = g[j];
   }
-  static constexpr int NA = 6;  // {w lm g_a, w G g_a}
+  static constexpr int NA = 6;  // {w lm g_a, w G g_a}: sa[j], sa[ND + j]
   template <int ND>
   __device__ static void side(const MatParams& mp, const float*, float w, const float* ga, float* sa) {
     const float G = mp.p[2], lm = mp.p[1] - 2.0f * G / 3.0f;
 #pragma unroll
-    for (int j = 0; j < 3; ++j) { sa[j] = w * lm * ga[j]; sa[3 + j] = w * G * ga[j]; }
+    for (int j = 0; j < ND; ++j) { sa[j] = w * lm * ga[j]; sa[ND + j] = w * G * ga[j]; }
   }
   template <int ND>
   __device__ static void pair(const float* sa, const float* gb, float* Kb) {
-    const float gg = fmaf(sa[3], gb[0], fmaf(sa[4], gb[1], sa[5] * gb[2]));
+    float gg = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int j = ND - 1; j >= 0; --j) gg = fmaf(sa[ND + j], gb[j], gg);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float t = fmaf(sa[i], gb[k], Kb[i * 3 + k]);
-        t = fmaf(gb[i], sa[3 + k], t);
-        Kb[i * 3 + k] = i == k ? t + gg : t;
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int k = 0; k < ND; ++k) {
+        float t = fmaf(sa[i], gb[k], Kb[i * ND + k]);
+        t = fmaf(gb[i], sa[ND + k], t);
+        Kb[i * ND + k] = i == k ? t + gg : t;
       }
   }
   template <int ND>
   __device__ static void dP(const MatParams& mp, const float*, const float* Gv, float* out) {
     const float G = mp.p[2], lm = mp.p[1] - 2.0f * G / 3.0f;
-    const float tr = Gv[0] + Gv[4] + Gv[8];
+    float tr = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < ND; ++i) tr += Gv[i * ND + i];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) out[i * 3 + j] = G * (Gv[i * 3 + j] + Gv[j * 3 + i]) + (i == j ? lm * tr : 0.0f);
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) out[i * ND + j] = G * (Gv[i * ND + j] + Gv[j * ND + i]) + (i == j ? lm * tr : 0.0f);
   }
 };
 

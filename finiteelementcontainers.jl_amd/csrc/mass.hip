@@ -204,6 +204,7 @@ int launch_mass(int kind, int nf, int what, const ElementArgs& a, float rho, hip
     if (kind == FECX_HEX8) { launch_mass_kind<FECX_HEX8, 1>(what, a, rho, s); return 0; }
     if (kind == FECX_TET10) { launch_mass_kind<FECX_TET10, 1>(what, a, rho, s); return 0; }
   }
+  if (nf == 2 && kind == FECX_QUAD4) { launch_mass_kind<FECX_QUAD4, 2>(what, a, rho, s); return 0; }
   if (nf == 3) {
     if (kind == FECX_HEX8) { launch_mass_kind<FECX_HEX8, 3>(what, a, rho, s); return 0; }
     if (kind == FECX_TET10) { launch_mass_kind<FECX_TET10, 3>(what, a, rho, s); return 0; }

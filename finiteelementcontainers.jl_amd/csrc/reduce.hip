@@ -638,6 +638,7 @@ int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s) {
     int64_t grid = (a.p1 - a.p0 + 255) / 256;
     grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8
     if (nf == 1) reduce_vector_kernel<1><<<(unsigned)grid, 256, 0, s>>>(a);
+    else if (nf == 2) reduce_vector_kernel<2><<<(unsigned)grid, 256, 0, s>>>(a);
     else if (nf == 3) reduce_vector_kernel<3><<<(unsigned)grid, 256, 0, s>>>(a);
     else return -1;
     return 0;
@@ -646,6 +647,8 @@ int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s) {
     if (a.nnpe == 4) return launch_buckets<1, 4>(a, s);
     if (a.nnpe == 8) return launch_buckets<1, 8>(a, s);
     if (a.nnpe == 10) return launch_buckets<1, 10>(a, s);
+  } else if (nf == 2) {
+    if (a.nnpe == 4) return launch_buckets<2, 4>(a, s);
   } else if (nf == 3) {
     if (a.nnpe == 8) return launch_buckets<3, 8>(a, s);
     if (a.nnpe == 10) return launch_buckets<3, 10>(a, s);

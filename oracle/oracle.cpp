@@ -238,6 +238,24 @@ void material_eval(const Material& m, int nd, int nf, const double* grad, const 
     return;
   }
   for (int i = 0; i < nf; ++i) src[i] = 0.0;
+  if (m.kind == LINEAR_ELASTIC && nd == 2) {
+    // plane strain (Mechanics(PlaneStrain()), test/mechanics/TestMechanics.jl:24): the 2 x 2 gradient embedded in a
+    // 3 x 3 one with a zero third row and column (modify_field_gradients, src/Formulations.jl:421-427), the 3D
+    // material evaluated on it, and its stress and tangent restricted to the in-plane components (extract_stress,
+    // extract_stiffness, Formulations.jl:391-419)
+    double g3[9] = {0}, P3[9], A3[81], s3[3];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) g3[i * 3 + j] = grad[i * 2 + j];
+    material_eval(m, 3, 3, g3, Xq, st_old, st_new, P3, s3, A ? A3 : nullptr);
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) P[i * 2 + j] = P3[i * 3 + j];
+    if (A)
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+          for (int k = 0; k < 2; ++k)
+            for (int l = 0; l < 2; ++l) A[(i * 2 + j) * 4 + k * 2 + l] = A3[(i * 3 + j) * 9 + k * 3 + l];
+    return;
+  }
   if (m.kind == LINEAR_ELASTIC) {
     // psi = K/2 tr(eps)^2 + G dev(eps):dev(eps)  (test/mechanics/TestMechanicsCommon.jl:14-20)
     const double K = m.p[1], G = m.p[2];
