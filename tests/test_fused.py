@@ -87,12 +87,22 @@ def _case(fec, name):
         return W.Workload("le", wl.mesh, fec.Mechanics(rho=1.0, K=3.0, G=1.0), 3, 0.01 * wl.U)
     if name == "j2 9^3 clamped":
         return W.j2_hex8(9, dirichlet=True)
+    if name.startswith("neohookean ") and name.count("x") == 2:  # thin and tiny bricks (node counts)
+        counts = tuple(int(c) for c in name.split()[1].split("x"))
+        wl = W.neohookean_hex8(2)
+        mesh = _brick(fec, counts)
+        # not the sine field, which vanishes on every node of a one-element-wide brick
+        return W.Workload("nh", mesh, wl.physics, 3, 0.02 * np.cos(1.3 * np.arange(3.0 * mesh.num_nodes)))
+    if name == "poisson 3x15x2":
+        mesh = _brick(fec, (3, 15, 2))
+        return W.Workload("p", mesh, fec.Poisson("constant", 1.0), 1, np.cos(np.arange(mesh.num_nodes, dtype=float)))
     raise KeyError(name)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["poisson 17x9x12", "neohookean 16^3", "neohookean 20x8x31", "mechanics 11^3",
-                                  "j2 9^3 clamped"])
+                                  "j2 9^3 clamped", "neohookean 2x2x2", "neohookean 9x2x2", "neohookean 2x3x17",
+                                  "neohookean 16x16x2", "poisson 3x15x2"])
 def test_fused_matches_two_phase_and_oracle(fec, orc, gpu, name):
     print(f"[fused] {name}", flush=True)
     wl = _case(fec, name)
