@@ -160,10 +160,10 @@ bool build_nodeinfo(fecx_ctx* ctx) {
 // assemblies of a one-block structured hex8 mesh with a plain material when asked for, or by default for the
 // vector-valued materials (on the scalar Poisson problem the two-phase path is faster: its element work is cheap and
 // the fused kernel's halo recompute and one-work-group-per-CU marching do not pay); the atomic and coloured designs
-// only where they are built (structured hex8, Poisson or neo-Hookean); the two-phase path everywhere else, and always
-// for the allocation-free entry point.
+// only where they are built (structured hex8, Poisson or neo-Hookean) and never for the allocation-free entry point
+// (their zero fill is a memset); the two-phase path everywhere else.
 int scatter_path(fecx_ctx* ctx, int mode, bool surrogate, bool noalloc) {
-  if (!ctx->structured || noalloc || surrogate || ctx->block_mp.size() != 1) return FECX_SCATTER_TWO_PHASE;
+  if (!ctx->structured || surrogate || ctx->block_mp.size() != 1) return FECX_SCATTER_TWO_PHASE;
   if (mode != (fecx::MODE_R | fecx::MODE_K) && mode != fecx::MODE_K) return FECX_SCATTER_TWO_PHASE;
   const int ph = ctx->mp.phys;
   const bool fused_ok = (ctx->nf == 1 && ph == FECX_POISSON) ||
@@ -171,7 +171,8 @@ int scatter_path(fecx_ctx* ctx, int mode, bool surrogate, bool noalloc) {
   const bool direct_ok = (ctx->nf == 1 && ph == FECX_POISSON) || (ctx->nf == 3 && ph == FECX_NEOHOOKEAN);
   if (ctx->scatter == FECX_SCATTER_FUSED && fused_ok) return FECX_SCATTER_FUSED;
   if (ctx->scatter == FECX_SCATTER_AUTO && fused_ok && ctx->nf == 3) return FECX_SCATTER_FUSED;
-  if ((ctx->scatter == FECX_SCATTER_ATOMIC || ctx->scatter == FECX_SCATTER_COLOURED) && direct_ok) return ctx->scatter;
+  if ((ctx->scatter == FECX_SCATTER_ATOMIC || ctx->scatter == FECX_SCATTER_COLOURED) && direct_ok && !noalloc)
+    return ctx->scatter;
   return FECX_SCATTER_TWO_PHASE;
 }
 
@@ -208,8 +209,9 @@ int read_flag(fecx_ctx* ctx, const int* flag) {
 // element chunks 0..j: element chunks run in order on stream2, reduction chunk j on the context's stream after an
 // event recorded behind element chunk j. Element chunk j + 1 (arithmetic-bound) overlaps reduction chunk j
 // (memory-bound), which reads element storage written moments earlier.
-// With `noalloc`, the element-private outputs go to the per-device scratch array (noalloc.hip) instead of the
-// context's buffers, and nothing is allocated: no profiling events, one piece, no surrogate pass.
+// With `noalloc`, nothing is allocated: no profiling events, one piece, no surrogate pass; the two-phase path keeps its
+// element-private outputs in the per-device scratch array (noalloc.hip) instead of the context's buffers, and the
+// fused kernel (structured hex8 meshes) needs none, so no scratch size limits it.
 void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double* R, double* vals,
               bool noalloc = false) {
   check_ready(ctx);
@@ -219,8 +221,9 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
   const bool surrogate_phys = ctx->mp.phys == FECX_J2_SURROGATE || ctx->mp.phys == FECX_NH_SURROGATE;
   const bool surrogate = surrogate_phys && ctx->sg_enabled;
   float *kel = get<float>(ctx, "Kel"), *rel = get<float>(ctx, "Rel");
-  if (noalloc) {
-    require(!surrogate, FECX_ERR_UNSUPPORTED, "the allocation-free entry point does not run surrogate materials");
+  const int path = scatter_path(ctx, mode, surrogate_phys, noalloc);
+  if (noalloc) require(!surrogate, FECX_ERR_UNSUPPORTED, "the allocation-free entry point does not run surrogate materials");
+  if (noalloc && path == FECX_SCATTER_TWO_PHASE) {  // the fused kernel keeps no element-private outputs
     const size_t kb = sizeof(float) * (size_t)ctx->ne * ctx->pairs.npairs * ctx->nf * ctx->nf;
     const size_t rb = sizeof(float) * (size_t)ctx->ne * ctx->nnpe * ctx->nf;
     const size_t kb_al = (kb + 255) / 256 * 256;
@@ -254,7 +257,6 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
     prof.run("surrogate_material", ctx->stream, [&] { fecx::surrogate_qp_pass(ctx, ea.hq, (float*)ea.qpmat, ctx->stream); });
     HIPCHK(hipGetLastError());
   }
-  const int path = scatter_path(ctx, mode, surrogate_phys, noalloc);
   if (path != FECX_SCATTER_TWO_PHASE) {
     fecx::FusedArgs fa{};
     fa.nx = ctx->snx; fa.ny = ctx->sny; fa.nz = ctx->snz;

@@ -185,12 +185,15 @@ int fecx_assemble_vector_matrix(fecx_ctx* ctx, const double* U, double* R, doubl
 
 /* fecx_assemble_vector_matrix that allocates nothing: the element-private outputs of its two-phase assembly go to a
  * file-scope device array of FECX_NOALLOC_SCRATCH_BYTES (64 MiB: about 48,000 hex8 or 31,000 tet10 elements with
- * three fields) that each GPU holds until the process ends, instead of the context's buffers. No counterpart in the
- * reference; it is for callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
- * harnesses that forbid hipMalloc). The context's own set-up (mesh, dofs, pattern, material) is done beforehand as
- * for every entry point. Meshes whose element storage exceeds the array get FECX_ERR_ARG; surrogate materials get
- * FECX_ERR_UNSUPPORTED. All calls on one GPU share the array, which keeps the last call's contents (every call
- * overwrites what it reads), so calls that overlap on one GPU -- from several threads or streams -- race. */
+ * three fields) that each GPU holds until the process ends, instead of the context's buffers; where the context takes
+ * the fused kernel (fecx_get_scatter_path: structured hex8 meshes), no element-private output exists and the mesh
+ * size is not limited. The atomic and coloured designs are not taken here (their zero fill is a memset). No
+ * counterpart in the reference; it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The context's own set-up (mesh, dofs, pattern, material) is
+ * done beforehand as for every entry point. Two-phase meshes whose element storage exceeds the array get
+ * FECX_ERR_ARG; surrogate materials get FECX_ERR_UNSUPPORTED. All calls on one GPU share the array, which keeps the
+ * last call's contents (every call overwrites what it reads), so calls that overlap on one GPU -- from several threads
+ * or streams -- race. */
 #define FECX_NOALLOC_SCRATCH_BYTES (64 << 20)
 int fecx_assemble_vector_matrix_noalloc(fecx_ctx* ctx, const double* U, double* R, double* vals);
 

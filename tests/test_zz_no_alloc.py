@@ -3,7 +3,8 @@ that cannot allocate device memory) against fecx_assemble_vector_matrix through 
 with the element-private outputs in the library's per-device scratch array, so the residual and the CSR values must
 agree bit for bit (the structured-hex8 fused kernel is switched off on these contexts so that both calls take the
 two-phase path). Calls in a row on the same GPU, each on a smaller mesh than the one before (the scratch keeps the
-previous call's contents), then a mesh too large for the scratch, which is refused before any launch."""
+previous call's contents), then a mesh too large for the scratch, which is refused before any launch; and the fused
+design, which needs no scratch, on a mesh beyond it."""
 import ctypes
 
 import numpy as np
@@ -49,6 +50,20 @@ def test_noalloc_matches_allocating_entry_point(fec, gpu, order):
         assert np.array_equal(R1, R2) and np.array_equal(K1, K2), name
         sizes.append(wl.mesh.num_elements * (wl.mesh.conn.shape[1] * wl.nf) ** 2)
     assert sizes[0] > sizes[1] > sizes[2]
+
+
+def test_noalloc_fused_beyond_the_scratch(fec, gpu):
+    """A structured hex8 mesh whose element storage would not fit the scratch (64,000 elements x 1,392 bytes), on
+    the fused design: no element-private outputs, so it runs, and agrees bit for bit with the allocating entry point on
+    the same design; then a smaller mesh in a row."""
+    W = fec.workloads
+    for name, wl in (("neohookean_hex8 40^3", W.neohookean_hex8(40)), ("neohookean_hex8 12^3", W.neohookean_hex8(12))):
+        print(f"[noalloc] fused {name}", flush=True)
+        asm, p = _ctx_for(fec, wl, gpu)
+        asm.ctx.call("fecx_set_scatter", ctypes.c_int(0))  # FECX_SCATTER_FUSED
+        R1, K1, R2, K2 = _both(fec, asm, p)
+        assert np.isfinite(R1).all() and np.isfinite(K1).all()
+        assert np.array_equal(R1, R2) and np.array_equal(K1, K2), name
 
 
 def test_noalloc_refuses_a_mesh_beyond_the_scratch(fec, gpu):
