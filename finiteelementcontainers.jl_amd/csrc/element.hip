@@ -735,7 +735,7 @@ union FusedWork {
 };
 
 template <int NF, int PHYS, int SX, int SY>
-__global__ __launch_bounds__(8 * SX * SY) __attribute__((amdgpu_waves_per_eu(2, 8)))
+__global__ __launch_bounds__(8 * SX * SY) __attribute__((amdgpu_waves_per_eu(NF == 1 ? 4 : 2, 8)))
 void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
   using TL = FusedTile<SX, SY>;
   using LDS = ElemLDS<FECX_HEX8, NF, PHYS, MODE_R | MODE_K>;
