@@ -71,7 +71,10 @@ dptr(f::FEC.AbstractField) = dptr(f.data)
 # reference's test tree, so the methods are added where those types are in scope):
 #   fecx_material(::Poisson, props)   = (FECX_POISSON, [0.0, 1.0])            # f = 1 (source kind 0)
 #   fecx_material(::Mechanics, props) = (FECX_LINEAR_ELASTIC, collect(props))  # {rho, K, G}
-# and a neo-Hookean or J2 physics of an application to FECX_NEOHOOKEAN {lambda, mu} / FECX_J2 {E, nu, sigma_y, H}.
+# (Mechanics(PlaneStrain()) on a quad4 mesh with two displacement fields maps the same way: the library takes the
+# plane-strain form from the two fields), and a neo-Hookean or J2 physics of an application to FECX_NEOHOOKEAN
+# {lambda, mu} / FECX_J2 {E, nu, sigma_y, H}. Dirichlet BCs on single components need nothing here: the DofManager's
+# dirichlet_dofs go to fecx_set_dofs as they are, and the pattern returned is the dof-level one.
 function fecx_material end
 fecx_material(physics, props) =
   throw(FECXError(Cint(-4), "no libfecx material registered for $(typeof(physics)); add a fecx_material method"))
