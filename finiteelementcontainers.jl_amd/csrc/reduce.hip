@@ -318,8 +318,8 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scatter_kernel(ReduceArgs 
   constexpr int VW = (NNPE + 3) / 4;               // vslot words per incidence
   constexpr int BW = NNPE * NB;                    // staged words per incidence
   constexpr int JN = (BW + LPR - 1) / LPR;         // words per lane and incidence
-  // incidences whose loads are in flight together: 2 on tet10 light rows (3 words per lane; C5 reduction 5.72 against
-  // 5.85 ms with 4), 4 on hex8 ones (C3 0.983 against 1.001 ms with 2)
+  // incidences whose loads are in flight together: 2 on tet10 light rows (3 words per lane; neo-Hookean tet10 80^3 x 6
+  // reduction 5.72 against 5.85 ms with 4), 4 on hex8 ones (neo-Hookean hex8 96^3: 0.983 against 1.001 ms with 2)
   constexpr int LB = JN <= 2 ? RED_LB1 : (NNPE == 10 ? 2 : RED_LB);
   static_assert(CHK <= LPR && CHK % LB == 0, "a row's lanes stage its incidences, LB per batch");
   __shared__ int16_t s_off[NNPE][BW];
