@@ -567,7 +567,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       int b, slot;
       bool tr;
       if (PJ::job(lane, p, b, slot, tr)) {
-        float* dst = Kblk + slot * (NF * NF);
+        float* dst = Kblk + pair_pos(NNPE, PJ::center(lane), b, slot) * (NF * NF);
 #pragma unroll
         for (int i = 0; i < NF; ++i)
 #pragma unroll
@@ -604,7 +604,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       float kv[NF] = {};
       for (int b = 0; b < NNPE; ++b) {
         const PairRef pr = pair_ref(NNPE, lane, b);
-        const float* blk = work.K + grp * KW + pr.slot * (NF * NF);
+        const float* blk = work.K + grp * KW + pair_pos(NNPE, lane, b, pr.slot) * (NF * NF);
 #pragma unroll
         for (int i = 0; i < NF; ++i)
 #pragma unroll

@@ -614,7 +614,8 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
         for (int j = 0; j < bw; ++j) {
           const int b = j / (nf * nf), c = j % (nf * nf);
           const fecx::PairRef pr = fecx::pair_ref(nnpe, la, b);
-          off[(size_t)la * bw + j] = (int16_t)(pr.slot * nf * nf + (pr.ab ? c : (c % nf) * nf + c / nf));
+          off[(size_t)la * bw + j] =
+              (int16_t)(fecx::pair_pos(nnpe, la, b, pr.slot) * nf * nf + (pr.ab ? c : (c % nf) * nf + c / nf));
         }
       upload(ctx, "kel_off", off);
     }

@@ -51,6 +51,22 @@ __host__ __device__ inline PairRef pair_ref(int nnpe, int a, int b) {
   return {pair_lane_off(nnpe, lo) + nnpe / 2, lo == a};
 }
 
+// Position of the block of the pair {a, b} (slot `slot` of the rule above) in an element's storage. tet10 (local
+// nodes 0-3 vertices, 4-9 edge midpoints): the edge-edge pairs, then the vertex-edge ones, then the vertex-vertex ones,
+// so that the rows of each node kind -- the reduction's two row buckets, launched one after the other -- read one
+// contiguous region of the record (blocks 0-44 and 21-54) instead of blocks spread over all of it. Other kinds: the
+// slot itself.
+__host__ __device__ inline int pair_pos(int nnpe, int a, int b, int slot) {
+  if (nnpe != 10) return slot;
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  if (lo >= 4) {
+    const int i = lo - 4, j = hi - 4;
+    return i * 6 - i * (i - 1) / 2 + (j - i);
+  }
+  if (hi >= 4) return 21 + lo * 6 + (hi - 4);
+  return 45 + lo * 4 - lo * (lo - 1) / 2 + (hi - lo);
+}
+
 // ------------------------------------------------------------------ host pattern
 // Block-CSR view of the reduced CSR pattern: a "free node" has all NF dofs unknown. Free nodes are
 // ranked in node order; the unknown index of (free rank r, dof i) is NF*r + i, so every CSR row of

@@ -152,12 +152,15 @@ __global__ __launch_bounds__(256) void mass_kernel(ElementArgs args, float rho) 
     __syncthreads();
     if (valid && lane < NNPE) {
       const int cnt = lane_cnt(NNPE, lane);
-      float* dst = work.K + grp * KW + pair_lane_off(NNPE, lane) * (NF * NF);
+      float* dst = work.K + grp * KW;
 #pragma unroll
       for (int p = 0; p < MAXP; ++p)
-        if (p < cnt)
+        if (p < cnt) {
+          const int b = lane + p < NNPE ? lane + p : lane + p - NNPE;
+          float* blk = dst + pair_pos(NNPE, lane, b, pair_lane_off(NNPE, lane) + p) * (NF * NF);
 #pragma unroll
-          for (int c = 0; c < NF * NF; ++c) dst[p * NF * NF + c] = (c % (NF + 1) == 0) ? kreg[p] : 0.0f;
+          for (int c = 0; c < NF * NF; ++c) blk[c] = (c % (NF + 1) == 0) ? kreg[p] : 0.0f;
+        }
     }
     __syncthreads();
     constexpr int EW = T::NPAIRS * NF * NF;
