@@ -164,11 +164,11 @@ __device__ __forceinline__ NodeIn<ND, NF> load_node_in(const ElementArgs& args, 
 // work.K, lane-major per element, element grp's at word grp * KS (KS = 0: LDS::KW), when this returns (after a
 // work-group barrier). Used by element_kernel (two-phase assembly) and by the structured hex8 kernels (which reduce
 // the blocks from LDS). `in` holds the lane's gather inputs (lanes below NNPE of a valid element; load_node_in), `sel`
-// the element id that indexes the quadrature-point state.
+// the element id that indexes the quadrature-point state; so_pre, if given, the old state of the lane's point (J2).
 template <int KIND, int NF, int PHYS, int MODE, int KS = 0, class LDS, class WORK, class RelOut>
 __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e, int64_t sel, bool valid,
                                              const NodeIn<ET<KIND>::ND, NF>& in, LDS* lds, WORK& work,
-                                             RelOut rel_out, bool& bad, int tid) {
+                                             RelOut rel_out, bool& bad, int tid, const float* so_pre = nullptr) {
   using T = ET<KIND>;
   using M = Mat<PHYS>;
   constexpr int ND = T::ND, NNPE = T::NNPE, NQ = T::NQ, G = T::G;
@@ -335,10 +335,10 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       for (int c = 0; c < M::NC; ++c) C[c] = qm[D + c];
     } else {
       float so[7] = {}, sn[7];
-      if constexpr (PHYS == FECX_J2) {
+      if constexpr (PHYS == FECX_J2) {  // the point's old state: loaded here, or already by the caller (so_pre)
         const int64_t sidx = (sel * NQ + q) * 7;
 #pragma unroll
-        for (int st = 0; st < 7; ++st) so[st] = args.st_old[sidx + st];
+        for (int st = 0; st < 7; ++st) so[st] = so_pre ? so_pre[st] : args.st_old[sidx + st];
       }
       M::template eval<ND>(args.mp, H, so, sn, P, C, bad);
       if constexpr (PHYS == FECX_J2) {
@@ -815,6 +815,18 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
       return load_node_in<3, NF>(args, ok ? nd : 0, ok, false);
     };
     NodeIn<3, NF> in = fetch(p0 - 1);
+    // J2: the old state of the lane's quadrature point (lane tid % 8) in its slot's element of layer k, fetched with the
+    // node inputs
+    float so[7] = {};
+    auto fetch_state = [&](int k) {
+      if constexpr (PHYS == FECX_J2) {
+        const bool ok = col_ok && k >= 0 && k < nez;
+        const int64_t sidx = ok ? ((((int64_t)ex * ney + ey) * nez + k) * 8 + tid % 8) * 7 : 0;
+#pragma unroll
+        for (int st = 0; st < 7; ++st) so[st] = ok ? args.st_old[sidx + st] : 0.0f;
+      }
+    };
+    fetch_state(p0 - 1);
 
     for (int k = p0 - 1; k < p1; ++k) {
       const bool layer_ok = k >= 0 && k < nez;
@@ -830,8 +842,10 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
       int tid_k = tid;
       asm volatile("" : "+v"(tid_k));
       element_eval<FECX_HEX8, NF, PHYS, MODE_R | MODE_K, KS>(args, 0, eid, valid, in, lds, work,
-                                                             [&](int w, float v) { srow[grp][w] = v; }, bad, tid_k);
+                                                             [&](int w, float v) { srow[grp][w] = v; }, bad, tid_k,
+                                                             PHYS == FECX_J2 ? so : nullptr);
       in = fetch(k + 1);  // the next layer's loads, in flight during the reduction
+      fetch_state(k + 1);
       double a0[NB], a1[NB], pm[NB];  // plane k: blocks dz = 0, +1; plane k + 1: block dz = -1
       if (task) {
 #pragma unroll
@@ -916,6 +930,10 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
       for (int d = 0; d < 3; ++d) asm volatile("" ::"v"(in.hi[d]), "v"(in.lo[d]));
 #pragma unroll
       for (int i = 0; i < NF; ++i) asm volatile("" ::"v"(in.u[i]));
+      if constexpr (PHYS == FECX_J2) {
+#pragma unroll
+        for (int st = 0; st < 7; ++st) asm volatile("" ::"v"(so[st]));
+      }
       // the plane's rows, node by node: each node's NB * deg words are contiguous in the CSR values
       {
         // wavefront w takes nodes w, w + W, ...; each node's rows in chunks of 64 words, every chunk's LDS read issued
