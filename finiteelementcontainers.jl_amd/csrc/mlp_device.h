@@ -106,9 +106,10 @@ struct MlpDev {
 constexpr float GELU_U = 1.5957691216057308f;   // 2 sqrt(2 / pi)
 constexpr float GELU_C = 0.044715f;
 constexpr float LOG2E = 1.4426950408889634f;
+// sigma(u) = 1 / (1 + 2^(-log2(e) u)) with the constants folded: -log2(e) u = z (GA + GB z^2)
+constexpr float GELU_A = -LOG2E * GELU_U, GELU_B = GELU_A * GELU_C;
 __device__ __forceinline__ float gelu_sigma(float z) {
-  const float u = GELU_U * z * fmaf(GELU_C * z, z, 1.0f);
-  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-LOG2E * u));
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(GELU_B, z * z, GELU_A)));
 }
 __device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }
 __device__ __forceinline__ float gelu_prime(float z) {
