@@ -87,6 +87,7 @@ _SIGS = {
     "fecx_j2_generate_samples": [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
                                  ctypes.c_void_p, ctypes.c_void_p],
     "fecx_set_chunks": [ctypes.c_void_p, ctypes.c_int],
+    "fecx_set_merged_rows": [ctypes.c_void_p, ctypes.c_int],
     "fecx_set_profiling": [ctypes.c_void_p, ctypes.c_int],
     "fecx_profile_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],
     "fecx_profile_entry": [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,

@@ -1,6 +1,7 @@
 // C-ABI entry points of libfecx.so (declared in include/fecx.h, each with its reference citation).
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -90,6 +91,8 @@ fecx::ReduceArgs reduce_args(fecx_ctx* ctx) {
   r.vent = get<int32_t>(ctx, "vent");
   r.vslot = get<uint8_t>(ctx, "vslot");
   r.kel_off = get<int16_t>(ctx, "kel_off");
+  r.merged = ctx->merged_rows ? get<int2>(ctx, "rmerge") : nullptr;
+  r.nmerged = (int64_t)(get_bytes(ctx, "rmerge") / sizeof(int2));
   r.Kel = get<float>(ctx, "Kel");
   r.Rel = get<float>(ctx, "Rel");
   return r;
@@ -326,6 +329,7 @@ void assemble(fecx_ctx* ctx, int mode, const double* U, const double* V, double*
     }
     ra.p0 = p0;
     ra.p1 = p1;
+    if (k > 1) ra.merged = nullptr;  // the merged launch's chunk table covers all rows
     prof.run("reduce", ctx->stream, [&] { rc = fecx::launch_reduce(ctx->nf, ra, ctx->stream); });
     require(rc == 0, FECX_ERR_UNSUPPORTED, "no reduction kernel for this element kind / nf");
     HIPCHK(hipGetLastError());
@@ -604,6 +608,25 @@ int fecx_build_pattern(fecx_ctx* ctx, int64_t* nnz) {
       upload(ctx, "rowinfo", ri);
       upload(ctx, "vent", pordered ? pv : hp.vent);
       upload(ctx, "vslot", pordered ? ps : hp.vslot);
+    }
+    {  // tet10 vector rows: the light and the heavy chunks of one launch, in the order of their first rows' Morton
+       // ranks, so that the vertex and edge rows of the same elements are reduced close together in time and share
+       // the element storage lines they read in L2
+      std::vector<int2> mg;
+      if (ctx->nnpe == 10 && nf == 3 && hp.nlight > 0 && hp.nlight < ctx->nn) {
+        const int64_t nl = hp.nlight, nh = ctx->nn - nl;
+        const int64_t cl = (nl + fecx::RED_MERGE_LIGHT - 1) / fecx::RED_MERGE_LIGHT;
+        const int64_t ch = (nh + fecx::RED_MERGE_HEAVY - 1) / fecx::RED_MERGE_HEAVY;
+        mg.reserve(cl + ch);
+        int64_t i = 0, j = 0;
+        while (i < cl || j < ch) {
+          const int32_t ki = i < cl ? hp.nrank[hp.norder[i * fecx::RED_MERGE_LIGHT]] : INT32_MAX;
+          const int32_t kj = j < ch ? hp.nrank[hp.norder[nl + j * fecx::RED_MERGE_HEAVY]] : INT32_MAX;
+          if (ki <= kj) mg.push_back(make_int2((int)(i++ * fecx::RED_MERGE_LIGHT), 0));
+          else mg.push_back(make_int2((int)(nl + j++ * fecx::RED_MERGE_HEAVY), 1));
+        }
+      }
+      upload(ctx, "rmerge", mg);
     }
     upload(ctx, "bptr", hp.bptr);
     upload(ctx, "bcol", hp.bcol);
@@ -912,6 +935,10 @@ int fecx_set_chunks(fecx_ctx* ctx, int chunks) {
     require(chunks >= 0 && chunks <= 1024, FECX_ERR_ARG, "chunks must be in [0, 1024]");
     ctx->chunks = chunks;
   });
+}
+
+int fecx_set_merged_rows(fecx_ctx* ctx, int on) {
+  return guard(ctx, [&] { ctx->merged_rows = on != 0; });
 }
 
 int fecx_set_profiling(fecx_ctx* ctx, int on) {

@@ -86,6 +86,7 @@ struct HostPattern {
   std::vector<int64_t> bptr;        // nfree + 1 : block row pointer
   std::vector<int32_t> bcol;        // nnzb: free rank of the column node, ascending per row
   std::vector<int32_t> norder;      // processing position -> node: light rows, then heavy rows, Morton order in each
+  std::vector<int32_t> nrank;       // node -> rank along the Morton curve
   int64_t nlight = 0;               // light rows: at most 8 incidences and 32 column blocks
   int maxdeg_light = 0;
   std::vector<int32_t> eperm;       // storage position -> element
@@ -172,12 +173,17 @@ struct ReduceArgs {
   const int32_t* vent;
   const uint8_t* vslot;
   const int16_t* kel_off;   // nnpe x (nnpe*nf*nf): word of (local node a, staged word j) in an element's storage
+  // tet10 rows of 3 fields, both buckets in one launch (reduce_merged_kernel): per work-group {first position, bucket},
+  // light chunks of RED_MERGE_LIGHT rows and heavy chunks of RED_MERGE_HEAVY rows, merged along the Morton curve
+  const int2* merged;
+  int64_t nmerged;
   const float* Kel;
   const float* Rel;
   double* vals;             // CSR values, or nullptr
   double* R;                // full NF x NN field, or nullptr
 };
 int launch_reduce(int nf, const ReduceArgs& a, hipStream_t s);
+constexpr int RED_MERGE_LIGHT = 8, RED_MERGE_HEAVY = 4;  // rows per work-group of the merged launch's two buckets
 
 // Allocation-free assembly (noalloc.hip): a file-scope device array of NOALLOC_SCRATCH_BYTES per GPU holds the
 // element-private outputs. noalloc_setup looks up the array's device address on `device` (which must be current) once;
@@ -258,6 +264,7 @@ struct fecx_ctx {
   bool sg_frozen_valid = false;
   // pipelined assembly: element chunks on stream2, reduction chunks on the context's stream
   int chunks = 0;  // 0: chosen from the element storage size
+  bool merged_rows = true;  // tet10 3-field reduction: both row buckets in one launch (fecx_set_merged_rows)
   hipStream_t stream2 = nullptr;
   std::vector<hipEvent_t> events;
   // numeric error flag: device word set by the kernels, read back through a pinned host word after every assembly;

@@ -85,7 +85,8 @@ void build_host_pattern(HostPattern& hp, int64_t nn, int64_t ne, int nnpe, const
   hp.nf = nf; hp.nnpe = nnpe; hp.nn = nn; hp.ne = ne;
   hp.vstride = (nnpe + 3) / 4 * 4;
   // Morton rank of every node (21 bits per coordinate over the bounding box), ties by node id
-  std::vector<int32_t> nrank(nn);
+  std::vector<int32_t>& nrank = hp.nrank;
+  nrank.assign(nn, 0);
   {
     double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
     for (int d = 0; d < nd; ++d) {

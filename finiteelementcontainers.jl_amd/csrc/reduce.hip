@@ -12,6 +12,8 @@
 //   reduce_scalar_hex8_kernel   scalar hex8 light rows: lanes add their words into LDS slot accumulators;
 //   reduce_scatter_kernel       3-field light rows of up to 32 blocks, and the tet10 heavy rows (scalar and 3-field)
 //                               of up to 72 blocks: the same scatter, 32 or 64 lanes per row;
+//   reduce_merged_kernel        3-field tet10 meshes: both buckets of reduce_scatter_kernel in one launch, their
+//                               work-groups interleaved along the Morton curve;
 //   reduce_rows_kernel          slot-owner lanes for everything else: light scalar tet10/quad4 rows, 3-field light
 //                               rows of more than 32 blocks, heavy hex8/quad4 rows and tet10 rows of more than 72;
 //   reduce_vector_kernel / reduce_vector_group_kernel   calls without CSR values (one thread, or 16 lanes, per row).
@@ -310,7 +312,19 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scalar_hex8_kernel(ReduceA
 // block's 9 FP64 adds on every lane whenever one lane matched: about 1,300 instructions per wavefront on the vertex
 // rows, which bound its time. Rows of more than SMAX blocks keep the slot-owner kernel.
 template <int NF, int NNPE, int LPR, int CHK, int SMAX>
-__global__ __launch_bounds__(RED_THREADS) void reduce_scatter_kernel(ReduceArgs a) {
+struct ScatterLDS {
+  static constexpr int NPW = RED_THREADS / LPR, NB = NF * NF, VW = (NNPE + 3) / 4, BW = NNPE * NB;
+  int16_t s_off[NNPE][BW];
+  int32_t s_v[NPW][CHK];
+  uint32_t s_w[NPW][CHK][VW];
+  float s_r[NPW][CHK][NF];
+  double s_acc[NPW][SMAX * NB];
+  int s_maxpass;
+};
+// The work-group's rows are the processing positions first + (thread / LPR), those below `end`.
+template <int NF, int NNPE, int LPR, int CHK, int SMAX>
+__device__ __forceinline__ void scatter_rows(const ReduceArgs& a, int64_t first, int64_t end, bool onepass,
+                                             ScatterLDS<NF, NNPE, LPR, CHK, SMAX>& L) {
   static_assert(LPR <= 64 && 64 % LPR == 0, "a row's lanes lie in one wavefront");
   constexpr int NPW = RED_THREADS / LPR;
   constexpr int NB = NF * NF;                      // words per block
@@ -322,30 +336,29 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scatter_kernel(ReduceArgs 
   // reduction 5.72 against 5.85 ms with 4), 4 on hex8 ones (neo-Hookean hex8 96^3: 0.983 against 1.001 ms with 2)
   constexpr int LB = JN <= 2 ? RED_LB1 : (NNPE == 10 ? 2 : RED_LB);
   static_assert(CHK <= LPR && CHK % LB == 0, "a row's lanes stage its incidences, LB per batch");
-  __shared__ int16_t s_off[NNPE][BW];
-  __shared__ int32_t s_v[NPW][CHK];
-  __shared__ uint32_t s_w[NPW][CHK][VW];
-  __shared__ float s_r[NPW][CHK][NF];
-  __shared__ double s_acc[NPW][SMAX * NB];
-  __shared__ int s_maxpass;
+  auto& s_off = L.s_off;
+  auto& s_v = L.s_v;
+  auto& s_w = L.s_w;
+  auto& s_r = L.s_r;
+  auto& s_acc = L.s_acc;
+  int& s_maxpass = L.s_maxpass;
   for (int i = threadIdx.x; i < NNPE * BW; i += RED_THREADS) (&s_off[0][0])[i] = a.kel_off[i];
-  const int64_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
-  const int64_t pos = a.p0 + chunk * NPW + grp;
-  const bool valid = pos < a.p1;
+  const int64_t pos = first + grp;
+  const bool valid = pos < end;
   const int4 ri = valid ? a.rowinfo[pos] : make_int4(0, 0, 0, 0);
   const int64_t n = ri.x;
   const int32_t c0 = ri.y, ninc = ri.w & 0xFFFF;
   const int deg = a.vals ? (ri.w >> 16) : 0;  // 0: no CSR blocks (a node with Dirichlet dofs)
   for (int k = lane; k < deg * NB; k += LPR) s_acc[grp][k] = 0.0;
   int maxpass = 1;  // light rows (at most CHK incidences) take one pass
-  if (!a.onepass) {
+  if (!onepass) {
     if (threadIdx.x == 0) s_maxpass = 0;
     __syncthreads();
     if (lane == 0) atomicMax(&s_maxpass, (ninc + CHK - 1) / CHK);
   }
   __syncthreads();  // s_off
-  if (!a.onepass) maxpass = s_maxpass;
+  if (!onepass) maxpass = s_maxpass;
   // this lane's words: block and entry of word jj * 64 + lane
   int wb[JN], wc[JN];
 #pragma unroll
@@ -423,7 +436,33 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scatter_kernel(ReduceArgs 
   }
 }
 
+template <int NF, int NNPE, int LPR, int CHK, int SMAX>
+__global__ __launch_bounds__(RED_THREADS) void reduce_scatter_kernel(ReduceArgs a) {
+  __shared__ ScatterLDS<NF, NNPE, LPR, CHK, SMAX> L;
+  constexpr int NPW = RED_THREADS / LPR;
+  scatter_rows<NF, NNPE, LPR, CHK, SMAX>(a, a.p0 + xcd_chunk(blockIdx.x, gridDim.x) * NPW, a.p1, a.onepass != 0, L);
+}
+
 constexpr int HEAVY_SCATTER_SMAX = 72;  // structured tet10 vertex rows: 65 blocks
+
+// tet10 rows of 3 fields, light (edge) and heavy (vertex) rows in one launch: work-group w takes merged chunk w (after
+// the XCD remap), RED_MERGE_LIGHT edge rows on 32 lanes each or RED_MERGE_HEAVY vertex rows on 64 lanes each (the two
+// bucket kernels' forms, LDS shared between them), the chunks ordered by their first rows' Morton ranks. A bucket's
+// own launch reads the blocks its rows share with the other bucket's rows (24 of an element's 55) a second time,
+// long after the first; here the vertex and edge rows of the same elements run at about the same time on the same
+// XCD. Every row's sums are those of the bucket kernels, so the bits are too.
+__global__ __launch_bounds__(RED_THREADS) void reduce_merged_kernel(ReduceArgs a) {
+  static_assert(RED_THREADS / 32 == RED_MERGE_LIGHT && RED_THREADS / 64 == RED_MERGE_HEAVY, "chunk sizes");
+  __shared__ union {
+    ScatterLDS<3, 10, 32, CH, 32> lo;
+    ScatterLDS<3, 10, 64, RED_HEAVY_CH, HEAVY_SCATTER_SMAX> hi;
+  } L;
+  const int64_t c = xcd_chunk(blockIdx.x, gridDim.x);
+  if (c >= a.nmerged) return;  // the grid is rounded up to a multiple of 8
+  const int2 ch = a.merged[c];
+  if (ch.y == 0) scatter_rows<3, 10, 32, CH, 32>(a, ch.x, a.nlight, false, L.lo);
+  else scatter_rows<3, 10, 64, RED_HEAVY_CH, HEAVY_SCATTER_SMAX>(a, ch.x, a.p1, false, L.hi);
+}
 
 // One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 8 for
 // scalar rows of up to 32 blocks (hex8: 27, in four sweeps; 32 rows per work-group, every lane busy in the gather of
@@ -488,6 +527,12 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
       rc = launch_lpr<NF, NNPE>(lo, a.maxdeg_light, s);
     }
   } else if constexpr (NF == 3) {
+    if (NNPE == 10 && a.merged && a.nmerged > 0 && a.p0 == 0 && a.nlight < a.p1 && a.maxdeg_light <= 32 &&
+        a.maxdeg <= HEAVY_SCATTER_SMAX) {
+      const int64_t grid = (a.nmerged + 7) / 8 * 8;  // XCD remap needs a multiple of 8
+      reduce_merged_kernel<<<(unsigned)grid, RED_THREADS, 0, s>>>(a);
+      return 0;
+    }
     if (lo.p1 > lo.p0 && a.maxdeg_light <= 32) {
       int64_t grid = (lo.p1 - lo.p0 + RED_THREADS / 32 - 1) / (RED_THREADS / 32);
       grid = (grid + 7) / 8 * 8;  // XCD remap needs a multiple of 8

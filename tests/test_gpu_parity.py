@@ -251,3 +251,35 @@ def test_edge_meshes(fec, orc, gpu, case):
     if len(unknown):
         assert rel(R, R_ref) < TOL["R"] and rel(K.values.cpu().numpy(), vals) < TOL["K"]
         assert rel(KV, KVm_ref) < TOL["KV"]
+
+
+@pytest.mark.parametrize("dirichlet", ["none", "random components"])
+def test_merged_row_launch_bit_identical(fec, gpu, dirichlet):
+    """tet10 with 3 fields: the edge and vertex rows reduced in one launch (fecx_set_merged_rows 1, the default) and
+    in two give the same residual and CSR values bit for bit."""
+    print(f"[gpu parity] merged rows, {dirichlet}", flush=True)
+    wl = fec.workloads.neohookean_tet10(4)
+    mesh = wl.mesh
+    V = fec.FunctionSpace(mesh, "H1Field", "Lagrange")
+    var = fec.VectorFunction(V, "displ")
+    asm = fec.SparseMatrixAssembler(var, sparse_matrix_type="csr", device=gpu.index or 0)
+    U = wl.U.copy()
+    bcs = []
+    if dirichlet != "none":
+        g = np.random.default_rng(7)
+        dd = np.sort(g.choice(3 * mesh.num_nodes, int(0.2 * 3 * mesh.num_nodes), replace=False) + 1)
+        for comp, name in enumerate(var.names):
+            nodes = (dd[(dd - 1) % 3 == comp] - 1) // 3 + 1
+            mesh.nodeset_nodes[f"_m{comp}"] = nodes
+            bcs.append(fec.DirichletBC(name, lambda X, t: 0.0, nset_name=f"_m{comp}"))
+        U = 0.2 * U
+        U[dd - 1] = 0.0
+    p = fec.create_parameters(mesh, asm, wl.physics, dirichlet_bcs=bcs)
+    Uu = torch.from_numpy(U[asm.dof.unknown_dofs - 1]).to(gpu)
+    out = []
+    for on in (1, 0, 1):
+        asm.ctx.call("fecx_set_merged_rows", ctypes.c_int(on))
+        fec.assemble_vector_and_stiffness_(asm, Uu, p)
+        out.append((fec.residual(asm).cpu().numpy().copy(), fec.stiffness(asm).values.cpu().numpy().copy()))
+    for R, K in out[1:]:
+        assert np.isfinite(K).all() and np.array_equal(R, out[0][0]) and np.array_equal(K, out[0][1])
