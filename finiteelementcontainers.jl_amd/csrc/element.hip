@@ -511,11 +511,11 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
   if constexpr (DO_K || DO_ASM) {
     if (valid && lane < PJ::LANES) {
       const int a = PJ::center(lane);
-      float acc[MAXP][NF * NF];
+      float acc[MAXP][NF * NF + M::KX];  // block words, then the material's extra accumulator (finish)
 #pragma unroll
       for (int p = 0; p < MAXP; ++p)
 #pragma unroll
-        for (int c = 0; c < NF * NF; ++c) acc[p][c] = 0.0f;
+        for (int c = 0; c < NF * NF + M::KX; ++c) acc[p][c] = 0.0f;
 #pragma unroll 2
       for (int q = 0; q < NQ; ++q) {
         const auto& Q = work.qp[grp].q[q];
@@ -535,6 +535,8 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
           }
         }
       }
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) M::template finish<ND>(acc[p]);
       if constexpr (NF > 1) {
         // the diagonal block K_aa (block 0 of lanes below NNPE) of these symmetric tangents is symmetric up to the
         // FP32 rounding of its two FMA orders; average it with its transpose so that the assembled Jacobian is
@@ -579,7 +581,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
     // diagonal of the element Jacobian (the AssembledDiagonal return type): lane a forms its own block K_aa only
     if (valid && lane < NNPE) {
       const int a = lane;
-      float acc[NF * NF] = {};
+      float acc[NF * NF + M::KX] = {};
 #pragma unroll 2
       for (int q = 0; q < NQ; ++q) {
         const auto& Q = work.qp[grp].q[q];
@@ -590,6 +592,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
         M::template side<ND>(args.mp, C, qp_weight(Q), va, sa);
         M::template pair<ND>(sa, va, acc);
       }
+      M::template finish<ND>(acc);
 #pragma unroll
       for (int i = 0; i < NF; ++i) rel_out(a * NF + i, acc[i * NF + i]);
     }
