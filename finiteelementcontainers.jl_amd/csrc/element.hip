@@ -508,6 +508,8 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       for (int i = 0; i < NF; ++i) rel_out(a * NF + i, out[i]);
     }
   }
+  // quad4 / hex8: block 0 of every lane is its diagonal block K_aa, formed by the material's cheaper symmetric form
+  constexpr bool DIAG0 = PJ::LANES == NNPE && M::DIAG;
   if constexpr (DO_K || DO_ASM) {
     if (valid && lane < PJ::LANES) {
       const int a = PJ::center(lane);
@@ -529,6 +531,12 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
           int b, slot;
           bool tr;
           if (PJ::job(lane, p, b, slot, tr)) {
+            if constexpr (DIAG0) {
+              if (p == 0) {
+                M::template pair_diag<ND>(sa, va, acc[0]);
+                continue;
+              }
+            }
             float vb[M::NV];
             node_vec(Q, C, b, vb);
             M::template pair<ND>(sa, vb, acc[p]);
@@ -539,14 +547,14 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       for (int p = 0; p < MAXP; ++p) M::template finish<ND>(acc[p]);
       if constexpr (NF > 1) {
         // the diagonal block K_aa (block 0 of lanes below NNPE) of these symmetric tangents is symmetric up to the
-        // FP32 rounding of its two FMA orders; average it with its transpose so that the assembled Jacobian is
-        // symmetric bit for bit
+        // FP32 rounding of its two FMA orders; average it with its transpose (or, formed as its upper triangle by
+        // pair_diag, mirror it) so that the assembled Jacobian is symmetric bit for bit
         if (lane < NNPE) {
 #pragma unroll
           for (int i = 0; i < NF; ++i)
 #pragma unroll
             for (int k = i + 1; k < NF; ++k) {
-              const float m = 0.5f * (acc[0][i * NF + k] + acc[0][k * NF + i]);
+              const float m = DIAG0 ? acc[0][i * NF + k] : 0.5f * (acc[0][i * NF + k] + acc[0][k * NF + i]);
               acc[0][i * NF + k] = m;
               acc[0][k * NF + i] = m;
             }
@@ -590,7 +598,8 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
         load_c(Q, C);
         node_vec(Q, C, a, va);
         M::template side<ND>(args.mp, C, qp_weight(Q), va, sa);
-        M::template pair<ND>(sa, va, acc);
+        if constexpr (DIAG0) M::template pair_diag<ND>(sa, va, acc);
+        else M::template pair<ND>(sa, va, acc);
       }
       M::template finish<ND>(acc);
 #pragma unroll

@@ -130,6 +130,7 @@ struct Mat<FECX_POISSON> {
   static constexpr int KX = 0;  // no extra accumulator word
   template <int ND>
   __device__ static void finish(float*) {}
+  static constexpr bool DIAG = false;  // no cheaper form of the block K_aa
   template <int ND>
   __device__ static void dP(const MatParams&, const float*, const float* Gv, float* out) {
 #pragma unroll
@@ -189,6 +190,20 @@ struct Mat<FECX_LINEAR_ELASTIC> {
   __device__ static void finish(float* Kb) {
 #pragma unroll
     for (int i = 0; i < ND; ++i) Kb[i * ND + i] += Kb[ND * ND];
+  }
+  // the block K_aa (gb = ga): the two rank-one terms are one, g_a (w (K - 2G/3) + w G) g_a^T, and symmetric: its upper
+  // triangle only (the caller mirrors it once the points are summed)
+  static constexpr bool DIAG = true;
+  template <int ND>
+  __device__ static void pair_diag(const float* sa, const float* ga, float* Kb) {
+    float gg = Kb[ND * ND];
+#pragma unroll
+    for (int j = ND - 1; j >= 0; --j) gg = fmaf(sa[ND + j], ga[j], gg);
+    Kb[ND * ND] = gg;
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int k = i; k < ND; ++k) Kb[i * ND + k] = fmaf(ga[i], sa[k] + sa[ND + k], Kb[i * ND + k]);
   }
   template <int ND>
   __device__ static void dP(const MatParams& mp, const float*, const float* Gv, float* out) {
@@ -278,6 +293,18 @@ struct Mat<FECX_NEOHOOKEAN> {
   __device__ static void finish(float* Kb) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) Kb[i * 3 + i] += Kb[9];
+  }
+  // the block K_aa (vb = va): the two rank-one terms are one, (c1 + lambda) w h_a h_a^T, and symmetric: its upper
+  // triangle only (the caller mirrors it once the points are summed)
+  static constexpr bool DIAG = true;
+  template <int ND>
+  __device__ static void pair_diag(const float* sa, const float* va, float* Kb) {
+    Kb[9] = fmaf(sa[0], va[0], fmaf(sa[1], va[1], fmaf(sa[2], va[2], Kb[9])));
+    const float* ha = va + 3;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = i; k < 3; ++k) Kb[i * 3 + k] = fmaf(ha[i], sa[3 + k] + sa[6 + k], Kb[i * 3 + k]);
   }
   template <int ND>
   __device__ static void dP(const MatParams& mp, const float* C, const float* Gv, float* out) {
@@ -389,6 +416,21 @@ struct Mat<FECX_J2> {
   static constexpr int KX = 0;
   template <int ND>
   __device__ static void finish(float*) {}
+  // the block K_aa (vb = va): g_a (w b1 + w b2) g_a^T - w b3 m_a m_a^T + the isotropic term, symmetric: its upper
+  // triangle only (the caller mirrors it once the points are summed)
+  static constexpr bool DIAG = true;
+  template <int ND>
+  __device__ static void pair_diag(const float* sa, const float* va, float* Kb) {
+    const float gg = fmaf(sa[3], va[0], fmaf(sa[4], va[1], sa[5] * va[2]));
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = i; k < 3; ++k) {
+        float t = fmaf(va[i], sa[k] + sa[3 + k], Kb[i * 3 + k]);
+        t = fmaf(sa[6 + i], va[3 + k], t);
+        Kb[i * 3 + k] = i == k ? t + gg : t;
+      }
+  }
   template <int ND>
   __device__ static void dP(const MatParams&, const float* C, const float* Gv, float* out) {
     const float b1 = C[6], b2 = C[7], b3 = C[8];
