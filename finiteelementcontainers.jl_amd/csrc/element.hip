@@ -103,8 +103,9 @@ __device__ __forceinline__ void scatter_points8(const float (&v)[8 * NF], int q,
 // Element-matrix blocks formed by one lane of the node phase. Every lane forms the blocks K_cb of one centre node c
 // (one evaluation of c's side terms per point). quad4 and hex8: c = lane, b = c + p for the lane's slots
 // (pair_lane_off order). tet10 (16 lanes, 55 blocks): lanes 0-9 form b = c + 0..3; lanes 10-15 form the 15 blocks of
-// offset 4 and 5 around c = lane - 10: K_c,c+4 and K_c,c+5 as stored, K_c,c+6 transposed (its slot holds K_c+6,c,
-// equal by the tangent's major symmetry). The busiest lane forms 4 blocks instead of 6.
+// offset 4 and 5 around c = lane - 10 as their blocks 1-3: K_c,c+4 and K_c,c+5 as stored, K_c,c+6 transposed (its
+// slot holds K_c+6,c, equal by the tangent's major symmetry). The busiest lane forms 4 blocks instead of 6, and block 0
+// of every lane that has one is its diagonal block K_cc.
 template <int KIND>
 struct PairJobs {
   static constexpr int NNPE = ET<KIND>::NNPE;
@@ -122,12 +123,14 @@ struct PairJobs {
       tr = false;
       return p < lim;
     }
-    const int d = p == 0 ? 4 : (p == 1 ? 6 : 5);
-    const int o = p == 1 ? c + 6 : c;  // the node whose slots hold the block
+    // no block at p = 0, where lanes 0-9 form their diagonal blocks (a cheaper form that then runs alone)
+    const int j = p - 1;
+    const int d = j == 0 ? 4 : (j == 1 ? 6 : 5);
+    const int o = j == 1 ? c + 6 : c;  // the node whose slots hold the block
     b = c + d < NNPE ? c + d : c + d - NNPE;
-    slot = pair_lane_off(NNPE, o) + (p == 2 ? 5 : 4);
-    tr = p == 1;
-    return p == 0 ? c <= 5 : (p == 1 ? c <= 3 : (p == 2 ? c <= 4 : false));
+    slot = pair_lane_off(NNPE, o) + (j == 2 ? 5 : 4);
+    tr = j == 1;
+    return j == 0 ? c <= 5 : (j == 1 ? c <= 3 : (j == 2 ? c <= 4 : false));
   }
 };
 
@@ -508,8 +511,9 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       for (int i = 0; i < NF; ++i) rel_out(a * NF + i, out[i]);
     }
   }
-  // quad4 / hex8: block 0 of every lane is its diagonal block K_aa, formed by the material's cheaper symmetric form
-  constexpr bool DIAG0 = PJ::LANES == NNPE && M::DIAG;
+  // block 0 of every lane that has one is its diagonal block K_aa (PairJobs), formed by the material's cheaper
+  // symmetric form
+  constexpr bool DIAG0 = M::DIAG;
   if constexpr (DO_K || DO_ASM) {
     if (valid && lane < PJ::LANES) {
       const int a = PJ::center(lane);
