@@ -73,11 +73,46 @@ constexpr int ELEM_THREADS = 256;  // four wavefronts, 256 / G elements per work
 // KS: words between two elements' blocks (LDS::KW, or more where a consumer wants another bank spread).
 template <class LDS, int EPW, int NQ, int KS = LDS::KW>
 struct ElemWork {
+  using Block = QPBlock<typename LDS::QP, NQ, qp_pad<typename LDS::QP, NQ>()>;
+  static constexpr bool WAVE_LOCAL = false;  // element blocks overlap other wavefronts' quadrature data
   union {
-    QPBlock<typename LDS::QP, NQ, qp_pad<typename LDS::QP, NQ>()> qp[EPW];
+    Block qp[EPW];
     float K[EPW * KS];
   };
+  __device__ Block& qpb(int e) { return qp[e]; }
+  __device__ float* kb(int e) { return K + e * KS; }
 };
+
+// The same per wavefront: each wavefront's quadrature data and, over them, its elements' blocks (contiguous, LDS::KW
+// words apart) in a region of their own, so that every LDS word an element's lanes write or read between the phases
+// belongs to their own wavefront: wavefront-local ordering replaces the work-group barriers (element_kernel).
+template <class LDS, int EPW, int NQ, int G>
+struct WaveWork {
+  using Block = QPBlock<typename LDS::QP, NQ, qp_pad<typename LDS::QP, NQ>()>;
+  static constexpr bool WAVE_LOCAL = true;
+  static constexpr int EPWAVE = 64 / G;
+  static_assert(EPW % EPWAVE == 0, "whole wavefronts of elements");
+  union Wave {
+    Block qp[EPWAVE];
+    float K[EPWAVE * LDS::KW];
+  };
+  Wave wv[EPW / EPWAVE];
+  static_assert(sizeof(Wave) % 16 == 0, "every wavefront's blocks start 16-byte aligned");
+  __device__ Block& qpb(int e) { return wv[e / EPWAVE].qp[e % EPWAVE]; }
+  __device__ float* kb(int e) { return wv[e / EPWAVE].K + (e % EPWAVE) * LDS::KW; }
+};
+
+// Ordering between the phases of element_eval: a work-group barrier, or (WAVE_LOCAL work areas) the program order of
+// one wavefront's LDS operations, which complete in order, kept by a wavefront-scope fence.
+template <class WORK>
+__device__ __forceinline__ void phase_sync() {
+  if constexpr (WORK::WAVE_LOCAL) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+}
 
 template <int CTRL>
 __device__ __forceinline__ float dpp_mov(float x) {
@@ -238,7 +273,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       }
     }
   }
-  __syncthreads();
+  phase_sync<WORK>();
 
   // ---------------- quadrature points: S lanes per point, lane S q + r handling the nodes r, r + S, r + 2S, ...
   // (tet10: 4 points on 16 lanes, S = 4; quad4/hex8: S = 1). The node sums (Jacobian, field gradient, source point,
@@ -297,7 +332,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
     }
     if (!(det > 0.0f)) bad = true;
-    auto& Q = work.qp[grp].q[q];
+    auto& Q = work.qpb(grp).q[q];
     float g[NMY][ND];
 #pragma unroll
     for (int k = 0; k < NMY; ++k)
@@ -479,7 +514,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
     }
     }  // !DO_G
   }
-  __syncthreads();
+  phase_sync<WORK>();
 
   // ---------------- node rows (lane = a)
   using PJ = PairJobs<KIND>;
@@ -491,7 +526,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       float out[NF] = {};
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        const auto& Q = work.qp[grp].q[q];
+        const auto& Q = work.qpb(grp).q[q];
         float Na = 0.0f, dNa[ND];
         if constexpr (DO_R && PHYS == FECX_POISSON) {
           float xi[ND];
@@ -524,7 +559,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
         for (int c = 0; c < NF * NF + M::KX; ++c) acc[p][c] = 0.0f;
 #pragma unroll 2
       for (int q = 0; q < NQ; ++q) {
-        const auto& Q = work.qp[grp].q[q];
+        const auto& Q = work.qpb(grp).q[q];
         float C[M::NC];
         float va[M::NV], sa[M::NA];
         load_c(Q, C);
@@ -596,7 +631,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       float acc[NF * NF + M::KX] = {};
 #pragma unroll 2
       for (int q = 0; q < NQ; ++q) {
-        const auto& Q = work.qp[grp].q[q];
+        const auto& Q = work.qpb(grp).q[q];
         float C[M::NC];
         float va[M::NV], sa[M::NA];
         load_c(Q, C);
@@ -613,14 +648,14 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
   if constexpr (DO_ASM) {
     // element action K_el v_el without atomics: stage the element's symmetric blocks in LDS (over the consumed
     // quadrature data), then lane a sums its row K_ab v_b over b in ascending order
-    __syncthreads();
-    if (valid && lane < PJ::LANES) stage_blocks(work.K + grp * KW);
-    __syncthreads();
+    phase_sync<WORK>();
+    if (valid && lane < PJ::LANES) stage_blocks(work.kb(grp));
+    phase_sync<WORK>();
     if (valid && lane < NNPE) {
       float kv[NF] = {};
       for (int b = 0; b < NNPE; ++b) {
         const PairRef pr = pair_ref(NNPE, lane, b);
-        const float* blk = work.K + grp * KW + pair_pos(NNPE, lane, b, pr.slot) * (NF * NF);
+        const float* blk = work.kb(grp) + pair_pos(NNPE, lane, b, pr.slot) * (NF * NF);
 #pragma unroll
         for (int i = 0; i < NF; ++i)
 #pragma unroll
@@ -632,9 +667,9 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
   }
   if constexpr (DO_K) {
     // stage the blocks in LDS, over the consumed quadrature data
-    __syncthreads();
-    if (valid && lane < PJ::LANES) stage_blocks(work.K + grp * KW);
-    __syncthreads();
+    phase_sync<WORK>();
+    if (valid && lane < PJ::LANES) stage_blocks(work.kb(grp));
+    phase_sync<WORK>();
   }
 }
 
@@ -646,7 +681,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   constexpr bool DO_K = LDS::DO_K;
   constexpr int KW = LDS::KW;
   __shared__ LDS lds[EPW];
-  __shared__ __attribute__((aligned(16))) ElemWork<LDS, EPW, NQ> work;
+  __shared__ __attribute__((aligned(16))) WaveWork<LDS, EPW, NQ, G> work;
   const int grp = threadIdx.x / G;
   const int64_t e = args.ebeg + (int64_t)blockIdx.x * EPW + grp;  // element storage position
   const bool valid = e < args.ne;
@@ -667,7 +702,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
     static_assert(EW == KW && (EPWAVE * EW) % 4 == 0, "a wavefront's blocks start 16-byte aligned in LDS");
     if (nvalid > 0) {
       float* out = args.Kel + e0 * EW;
-      const float* src = work.K + wv * EPWAVE * EW;
+      const float* src = work.kb(wv * EPWAVE);
       const int total = (int)nvalid * EW;
       int done = 0;
       // nontemporal: the blocks are read once, by the reduction (measured -2.5 % on the reduction that follows)
