@@ -199,9 +199,9 @@ __device__ __forceinline__ NodeIn<ND, NF> load_node_in(const ElementArgs& args, 
 // The element computation of one 256-thread work-group, G lanes per element (gather, quadrature points, node phase).
 // Lane group grp = threadIdx.x / G evaluates the element at storage position e (when `valid`). Every element row
 // (residual, action, diagonal) leaves through rel_out(word a * NF + i, value); the element blocks (DO_K) are left in
-// work.K, lane-major per element, element grp's at word grp * KS (KS = 0: LDS::KW), when this returns (after a
-// work-group barrier). Used by element_kernel (two-phase assembly) and by the structured hex8 kernels (which reduce
-// the blocks from LDS). `in` holds the lane's gather inputs (lanes below NNPE of a valid element; load_node_in), `sel`
+// the work area, element grp's at work.kb(grp) in its storage layout, when this returns (after a work-group barrier,
+// or for a wavefront-local work area an ordering of the wavefront's own LDS operations). Used by element_kernel
+// (two-phase assembly, WaveWork) and by the structured hex8 kernels (ElemWork; they reduce the blocks from LDS). `in` holds the lane's gather inputs (lanes below NNPE of a valid element; load_node_in), `sel`
 // the element id that indexes the quadrature-point state; so_pre, if given, the old state of the lane's point (J2).
 template <int KIND, int NF, int PHYS, int MODE, int KS = 0, class LDS, class WORK, class RelOut>
 __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e, int64_t sel, bool valid,
