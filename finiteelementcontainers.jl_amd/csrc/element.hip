@@ -810,7 +810,11 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
   const bool task = tid < TL::TASKS;
   const bool res_lane = tid >= TL::RES0 && tid < TL::RES0 + TL::NODES;
   const int node = task ? tid % TL::NODES : (res_lane ? tid - TL::RES0 : 0);
-  const int dxdy = task ? tid / TL::NODES : 4;
+  // the lane groups' (dx, dy) in an order whose wavefronts straddle groups with overlapping candidate slots: a
+  // wavefront runs the union of its lanes' candidates, 16 candidate rounds per layer in this order against 18 in
+  // (dx, dy) order (swapping (-1, 0) and (1, 0))
+  const int grp9 = tid / TL::NODES;
+  const int dxdy = task ? (grp9 == 3 ? 5 : (grp9 == 5 ? 3 : grp9)) : 4;
   const int li = node % TL::TX, lj = node / TL::TX;
   const int dx = dxdy % 3 - 1, dy = dxdy / 3 - 1;
   const int dm = (dx + 1) + 3 * (dy + 1);  // neighbour index of (dx, dy, -1); + 9 and + 18 for dz = 0 and + 1
