@@ -427,6 +427,8 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the full-size parity and CPU legs (profiling runs)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg (J2, tet10 NH)")
     ap.add_argument("--no-operators", action="store_true", help="skip the timings of the other entry points")
+    ap.add_argument("--fp64-geometry", action="store_true",
+                    help="tet10: element geometry, field gradient and stress in FP64 (fecx_set_fp64_geometry)")
     ap.add_argument("--newton", type=int, default=0, metavar="N",
                     help="j2_hex8 learned leg: also solve the 20 load steps on N^3 elements by Newton, plain and "
                          "with the surrogate (N = 64: the configuration's full size)")
@@ -469,6 +471,8 @@ def main():
     import ctypes
     design = ["fused", "two_phase", "atomic", "coloured", "auto"].index(args.scatter)
     asm.ctx.call("fecx_set_scatter", ctypes.c_int(design))
+    asm.ctx.call("fecx_set_fp64_geometry", ctypes.c_int(1 if args.fp64_geometry else 0))
+    fp64_geo = args.fp64_geometry and wl.mesh.element_type == "TET10"
     path = ctypes.c_int(-1)
     asm.ctx.call("fecx_get_scatter_path", ctypes.byref(path))
     scatter_path = ["fused", "two_phase", "atomic", "coloured"][path.value]
@@ -528,7 +532,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{name} {n}^3" if wl.mesh.num_dims == 3 else f"{name} {n}^2",
                    "elements": int(wl.num_elements), "nnz": int(asm.nnz), "step": "residual + CSR Jacobian",
-                   "element_math": "fp32", "reductions": "fp64", "chunks": max(1, args.chunks),
+                   "element_math": "fp32, geometry and stress fp64" if fp64_geo else "fp32", "reductions": "fp64",
+                   "chunks": max(1, args.chunks),
                    "scatter": scatter_path},
         "kernels": kernels,
     }

@@ -88,6 +88,7 @@ _SIGS = {
                                  ctypes.c_void_p, ctypes.c_void_p],
     "fecx_set_chunks": [ctypes.c_void_p, ctypes.c_int],
     "fecx_set_merged_rows": [ctypes.c_void_p, ctypes.c_int],
+    "fecx_set_fp64_geometry": [ctypes.c_void_p, ctypes.c_int],
     "fecx_set_profiling": [ctypes.c_void_p, ctypes.c_int],
     "fecx_profile_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],
     "fecx_profile_entry": [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,

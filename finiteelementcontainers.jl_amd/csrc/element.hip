@@ -20,8 +20,18 @@
 namespace fecx {
 
 // ------------------------------------------------------------------ the kernel
-template <int KIND, int NF, int PHYS, int MODE>
-struct ElemLDS {
+// The element's coordinate and field differences to its local node 0 (in the element record, or, IN_WORK, elsewhere)
+template <class RT, int NNPE, int ND, int NF, bool IN_WORK>
+struct NodeDiffs {
+  RT x[NNPE][ND];
+  RT u[NNPE][NF];
+};
+template <class RT, int NNPE, int ND, int NF>
+struct NodeDiffs<RT, NNPE, ND, NF, true> {};
+
+template <int KIND, int NF, int PHYS, int MODE, bool G64 = false>
+struct ElemLDS : NodeDiffs<std::conditional_t<G64, double, float>, ET<KIND>::NNPE, ET<KIND>::ND, NF,
+                           KIND == FECX_TET10> {
   using T = ET<KIND>;
   static constexpr bool DO_R = (MODE & MODE_R) != 0, DO_K = (MODE & MODE_K) != 0;
   static constexpr bool DO_MF = (MODE & MODE_ACTION_MF) != 0, DO_ASM = (MODE & MODE_ACTION_ASM) != 0;
@@ -44,9 +54,15 @@ struct ElemLDS {
   static constexpr bool SC = HEX && PHYS == FECX_POISSON;
   static constexpr bool QR = HEX && DO_R, QM = HEX && DO_MF;
   static constexpr bool NOQP = HEX && !(DO_K || DO_ASM || DO_D);  // no node-phase consumer of quadrature data
-  float x[T::NNPE][T::ND];
+  // G64 (tet10, fecx_set_fp64_geometry): the element geometry, field gradient and stress are evaluated in FP64 (from
+  // FP64 coordinate and field differences); their FP32 roundings feed the tangent, the residual contraction and the
+  // stored rows. tet10: the differences (Geo) live in the wavefront's work area beside the quadrature data (GW), in
+  // space that the element blocks take only once both are consumed.
+  static_assert(!G64 || KIND == FECX_TET10, "FP64 element geometry for tet10 only");
+  using RT = std::conditional_t<G64, double, float>;
+  static constexpr bool GW = KIND == FECX_TET10;
+  using Geo = NodeDiffs<RT, T::NNPE, T::ND, NF, false>;
   float x0[T::ND];
-  float u[T::NNPE][NF];
   float v[(DO_MF || DO_ASM) ? T::NNPE : 1][NF];
   struct QPFull {
     float v[T::NNPE][(DO_K || DO_ASM || DO_D) && !GS ? Mat<PHYS>::NV : T::ND];  // v[a][0..ND) = shape gradient g_a
@@ -75,6 +91,7 @@ template <class LDS, int EPW, int NQ, int KS = LDS::KW>
 struct ElemWork {
   using Block = QPBlock<typename LDS::QP, NQ, qp_pad<typename LDS::QP, NQ>()>;
   static constexpr bool WAVE_LOCAL = false;  // element blocks overlap other wavefronts' quadrature data
+  static_assert(!LDS::GW, "element differences in the element records");
   union {
     Block qp[EPW];
     float K[EPW * KS];
@@ -92,14 +109,22 @@ struct WaveWork {
   static constexpr bool WAVE_LOCAL = true;
   static constexpr int EPWAVE = 64 / G;
   static_assert(EPW % EPWAVE == 0, "whole wavefronts of elements");
-  union Wave {
+  struct QPOnly {
     Block qp[EPWAVE];
+  };
+  struct QPGeo {
+    Block qp[EPWAVE];
+    typename LDS::Geo geo[EPWAVE];
+  };
+  union Wave {
+    std::conditional_t<LDS::GW, QPGeo, QPOnly> a;
     float K[EPWAVE * LDS::KW];
   };
   Wave wv[EPW / EPWAVE];
   static_assert(sizeof(Wave) % 16 == 0, "every wavefront's blocks start 16-byte aligned");
-  __device__ Block& qpb(int e) { return wv[e / EPWAVE].qp[e % EPWAVE]; }
+  __device__ Block& qpb(int e) { return wv[e / EPWAVE].a.qp[e % EPWAVE]; }
   __device__ float* kb(int e) { return wv[e / EPWAVE].K + (e % EPWAVE) * LDS::KW; }
+  __device__ typename LDS::Geo& geo(int e) { return wv[e / EPWAVE].a.geo[e % EPWAVE]; }
 };
 
 // Ordering between the phases of element_eval: a work-group barrier, or (WAVE_LOCAL work areas) the program order of
@@ -117,6 +142,13 @@ __device__ __forceinline__ void phase_sync() {
 template <int CTRL>
 __device__ __forceinline__ float dpp_mov(float x) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_f64(double x) {  // the same permutation of both 32-bit halves
+  const int64_t b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
 }
 
 // Reduce-scatter over the 8 quadrature-point lanes of a hex8 element. On entry each lane holds v[a * NF + i], its
@@ -216,9 +248,19 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
   constexpr bool NEED_C = LDS::NEED_C, GS = LDS::GS, SC = LDS::SC, QR = LDS::QR, QM = LDS::QM, NOQP = LDS::NOQP;
   constexpr int KW = KS ? KS : LDS::KW;
   static_assert(KW >= LDS::KW, "element block stride below the element's words");
+  using RT = typename LDS::RT;  // the scalar of the element geometry and stress chain
 
   const int lane = tid % G, grp = tid / G;
   auto& L = lds[grp];
+  // node a's coordinate and field differences to local node 0
+  auto Xn = [&](int a) -> RT* {
+    if constexpr (LDS::GW) return work.geo(grp).x[a];
+    else return L.x[a];
+  };
+  auto Un = [&](int a) -> RT* {
+    if constexpr (LDS::GW) return work.geo(grp).u[a];
+    else return L.u[a];
+  };
 
   // The node lanes' view of quadrature point Q: its tangent data C (the side scalars C[S0 .. S0+NS), or all of it in
   // the compact form), and node b's vector vb (staged, or formed here from g_b by the same prep as the staged form).
@@ -257,13 +299,17 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       const float h0 = __shfl(hi[d], src0), l0 = __shfl(lo[d], src0);
-      if (lane < NNPE) L.x[lane][d] = (hi[d] - h0) + (lo[d] - l0);
+      if constexpr (std::is_same_v<RT, double>) {
+        if (lane < NNPE) Xn(lane)[d] = ((double)hi[d] - (double)h0) + ((double)lo[d] - (double)l0);
+      } else {
+        if (lane < NNPE) Xn(lane)[d] = (hi[d] - h0) + (lo[d] - l0);
+      }
       if (lane == 0) L.x0[d] = h0 + l0;
     }
 #pragma unroll
     for (int i = 0; i < NF; ++i) {
       const double u0 = __shfl(uu[i], src0);
-      if (lane < NNPE) L.u[lane][i] = (float)(uu[i] - u0);
+      if (lane < NNPE) Un(lane)[i] = (RT)(uu[i] - u0);
       if constexpr (DO_MF) {
         const double v0 = __shfl(vv[i], src0);
         if (lane < NNPE) L.v[lane][i] = (float)(vv[i] - v0);
@@ -283,47 +329,52 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
   constexpr int S = KIND == FECX_TET10 ? G / NQ : 1;
   constexpr int NMY = (NNPE + S - 1) / S;
   static_assert(NQ * S <= G && (S == 1 || S == 4), "the lanes of an element cover its quadrature points in quads");
-  auto lane_sum = [](float& v) {
+  auto lane_sum = [](auto& v) {
     if constexpr (S == 4) {
-      v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
-      v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+      if constexpr (std::is_same_v<std::decay_t<decltype(v)>, double>) {
+        v += dpp_mov_f64<0xB1>(v);  // quad_perm 1,0,3,2
+        v += dpp_mov_f64<0x4E>(v);  // quad_perm 2,3,0,1
+      } else {
+        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
+        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+      }
     }
   };
   if (valid && lane < NQ * S) {
     const int q = lane / S, r = lane % S;
     auto mine = [&](int k) { return S == 1 || r + S * k < NNPE; };
-    float xi[ND];
+    RT xi[ND];
     qpoint<KIND>(q, xi);
-    float N[NMY], dN[NMY][ND];
+    RT N[NMY], dN[NMY][ND];
 #pragma unroll
     for (int k = 0; k < NMY; ++k) {
-      N[k] = 0.0f;
+      N[k] = RT(0);
 #pragma unroll
-      for (int j = 0; j < ND; ++j) dN[k][j] = 0.0f;
+      for (int j = 0; j < ND; ++j) dN[k][j] = RT(0);
       if (mine(k)) shape_node<KIND>(r + S * k, xi, N[k], dN[k]);
     }
-    float J[ND][ND] = {};
+    RT J[ND][ND] = {};
 #pragma unroll
     for (int k = 0; k < NMY; ++k)
       if (mine(k))
 #pragma unroll
         for (int i = 0; i < ND; ++i)
 #pragma unroll
-          for (int j = 0; j < ND; ++j) J[i][j] += L.x[r + S * k][i] * dN[k][j];
+          for (int j = 0; j < ND; ++j) J[i][j] += Xn(r + S * k)[i] * dN[k][j];
 #pragma unroll
     for (int i = 0; i < ND; ++i)
 #pragma unroll
       for (int j = 0; j < ND; ++j) lane_sum(J[i][j]);
-    float Ji[ND][ND], det;
+    RT Ji[ND][ND], det;
     if constexpr (ND == 2) {
       det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-      const float id = 1.0f / det;
+      const RT id = RT(1) / det;
       Ji[0][0] = J[1][1] * id; Ji[0][1] = -J[0][1] * id; Ji[1][0] = -J[1][0] * id; Ji[1][1] = J[0][0] * id;
     } else {
-      const float c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
-                  c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      const RT c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+               c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
       det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-      const float id = 1.0f / det;
+      const RT id = RT(1) / det;
       Ji[0][0] = c00 * id; Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
       Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
       Ji[1][0] = c01 * id; Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
@@ -331,32 +382,41 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       Ji[2][0] = c02 * id; Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
       Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
     }
-    if (!(det > 0.0f)) bad = true;
+    if (!(det > RT(0))) bad = true;
     auto& Q = work.qpb(grp).q[q];
-    float g[NMY][ND];
+    RT gr[NMY][ND];
 #pragma unroll
     for (int k = 0; k < NMY; ++k)
 #pragma unroll
       for (int j = 0; j < ND; ++j) {
-        float t = 0;
+        RT t = 0;
 #pragma unroll
         for (int i = 0; i < ND; ++i) t += dN[k][i] * Ji[i][j];
-        g[k][j] = t;
+        gr[k][j] = t;
       }
-    const float w = det * qweight<KIND>();
-    if constexpr (!SC && !NOQP) {
-      if (r == 0) Q.w = w;
-    }
-    float H[D] = {};
+    const RT wr = det * qweight<KIND, RT>();
+    RT Hr[D] = {};
 #pragma unroll
     for (int k = 0; k < NMY; ++k)
       if (mine(k))
 #pragma unroll
         for (int i = 0; i < NF; ++i)
 #pragma unroll
-          for (int j = 0; j < ND; ++j) H[i * ND + j] += L.u[r + S * k][i] * g[k][j];
+          for (int j = 0; j < ND; ++j) Hr[i * ND + j] += Un(r + S * k)[i] * gr[k][j];
 #pragma unroll
-    for (int c = 0; c < D; ++c) lane_sum(H[c]);
+    for (int c = 0; c < D; ++c) lane_sum(Hr[c]);
+    // FP32 shape gradients, weight and field gradient from here on (the same values where RT is float)
+    float g[NMY][ND], H[D];
+#pragma unroll
+    for (int k = 0; k < NMY; ++k)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) g[k][j] = (float)gr[k][j];
+#pragma unroll
+    for (int c = 0; c < D; ++c) H[c] = (float)Hr[c];
+    const float w = (float)wr;
+    if constexpr (!SC && !NOQP) {
+      if (r == 0) Q.w = w;
+    }
     if constexpr (DO_G) {  // quadrature-point gradients only, element storage order
       if (r == 0) {
         float* o = args.hq + ((int64_t)e * NQ + q) * D;
@@ -378,7 +438,10 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
 #pragma unroll
         for (int st = 0; st < 7; ++st) so[st] = so_pre ? so_pre[st] : args.st_old[sidx + st];
       }
-      M::template eval<ND>(args.mp, H, so, sn, P, C, bad);
+      RT Pr[D];
+      M::template eval<ND>(args.mp, Hr, so, sn, Pr, C, bad);
+#pragma unroll
+      for (int c = 0; c < D; ++c) P[c] = (float)Pr[c];
       if constexpr (PHYS == FECX_J2) {
         if (DO_R && args.st_new && r == 0) {
           const int64_t sidx = (sel * NQ + q) * 7;
@@ -404,12 +467,12 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
           f *= (float)ND * 9.8696044010893586f;
 #pragma unroll
           for (int d = 0; d < ND; ++d) {
-            float t = 0;
+            RT t = 0;
 #pragma unroll
             for (int k = 0; k < NMY; ++k)
-              if (mine(k)) t += N[k] * L.x[r + S * k][d];
+              if (mine(k)) t += N[k] * Xn(r + S * k)[d];
             lane_sum(t);
-            f *= sinpif(L.x0[d] + t);
+            f *= sinpif(L.x0[d] + (float)t);
           }
         }
         if constexpr (!QR) {
@@ -673,10 +736,10 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
   }
 }
 
-template <int KIND, int NF, int PHYS, int MODE>
+template <int KIND, int NF, int PHYS, int MODE, bool G64 = false>
 __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   using T = ET<KIND>;
-  using LDS = ElemLDS<KIND, NF, PHYS, MODE>;
+  using LDS = ElemLDS<KIND, NF, PHYS, MODE, G64>;
   constexpr int NNPE = T::NNPE, NQ = T::NQ, G = T::G, EPW = ELEM_THREADS / G, NDOF = NNPE * NF;
   constexpr bool DO_K = LDS::DO_K;
   constexpr int KW = LDS::KW;
@@ -717,7 +780,7 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
   if (bad) atomicOr(args.flag, 1);
 }
 
-template <int KIND, int NF, int PHYS>
+template <int KIND, int NF, int PHYS, bool G64 = false>
 static void launch_modes(int mode, const ElementArgs& a, hipStream_t s) {
   using T = ET<KIND>;
   constexpr int EPW = ELEM_THREADS / T::G;
@@ -725,14 +788,14 @@ static void launch_modes(int mode, const ElementArgs& a, hipStream_t s) {
   const unsigned grid = (unsigned)((a.ne - a.ebeg + EPW - 1) / EPW);
   constexpr int B = ELEM_THREADS;
   switch (mode) {
-    case MODE_R: element_kernel<KIND, NF, PHYS, MODE_R><<<grid, B, 0, s>>>(a); break;
-    case MODE_K: element_kernel<KIND, NF, PHYS, MODE_K><<<grid, B, 0, s>>>(a); break;
-    case MODE_R | MODE_K: element_kernel<KIND, NF, PHYS, MODE_R | MODE_K><<<grid, B, 0, s>>>(a); break;
-    case MODE_ACTION_MF: element_kernel<KIND, NF, PHYS, MODE_ACTION_MF><<<grid, B, 0, s>>>(a); break;
-    case MODE_ACTION_ASM: element_kernel<KIND, NF, PHYS, MODE_ACTION_ASM><<<grid, B, 0, s>>>(a); break;
-    case MODE_DIAG: element_kernel<KIND, NF, PHYS, MODE_DIAG><<<grid, B, 0, s>>>(a); break;
+    case MODE_R: element_kernel<KIND, NF, PHYS, MODE_R, G64><<<grid, B, 0, s>>>(a); break;
+    case MODE_K: element_kernel<KIND, NF, PHYS, MODE_K, G64><<<grid, B, 0, s>>>(a); break;
+    case MODE_R | MODE_K: element_kernel<KIND, NF, PHYS, MODE_R | MODE_K, G64><<<grid, B, 0, s>>>(a); break;
+    case MODE_ACTION_MF: element_kernel<KIND, NF, PHYS, MODE_ACTION_MF, G64><<<grid, B, 0, s>>>(a); break;
+    case MODE_ACTION_ASM: element_kernel<KIND, NF, PHYS, MODE_ACTION_ASM, G64><<<grid, B, 0, s>>>(a); break;
+    case MODE_DIAG: element_kernel<KIND, NF, PHYS, MODE_DIAG, G64><<<grid, B, 0, s>>>(a); break;
     case MODE_QPGRAD:  // gradients do not depend on the physics: one instantiation per (KIND, NF)
-      element_kernel<KIND, NF, NF == 1 ? FECX_POISSON : FECX_NEOHOOKEAN, MODE_QPGRAD><<<grid, B, 0, s>>>(a);
+      element_kernel<KIND, NF, NF == 1 ? FECX_POISSON : FECX_NEOHOOKEAN, MODE_QPGRAD, G64><<<grid, B, 0, s>>>(a);
       break;
     default: break;
   }
@@ -1147,12 +1210,19 @@ int launch_fused_hex8(int nf, const ElementArgs& a, const FusedArgs& f, hipStrea
   return -1;
 }
 
+// tet10 in FP32 or with the FP64 element geometry (ElementArgs::fp64_geometry)
+template <int NF, int PHYS>
+static void launch_tet10(int mode, const ElementArgs& a, hipStream_t s) {
+  if (a.fp64_geometry) launch_modes<FECX_TET10, NF, PHYS, true>(mode, a, s);
+  else launch_modes<FECX_TET10, NF, PHYS, false>(mode, a, s);
+}
+
 int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t s) {
   const int phys = a.mp.phys;
   if (nf == 1 && phys == FECX_POISSON) {
     if (kind == FECX_QUAD4) { launch_modes<FECX_QUAD4, 1, FECX_POISSON>(mode, a, s); return 0; }
     if (kind == FECX_HEX8) { launch_modes<FECX_HEX8, 1, FECX_POISSON>(mode, a, s); return 0; }
-    if (kind == FECX_TET10) { launch_modes<FECX_TET10, 1, FECX_POISSON>(mode, a, s); return 0; }
+    if (kind == FECX_TET10) { launch_tet10<1, FECX_POISSON>(mode, a, s); return 0; }
     return -1;
   }
   if (nf == 2 && kind == FECX_QUAD4 && phys == FECX_LINEAR_ELASTIC) {  // plane strain
@@ -1167,11 +1237,11 @@ int launch_element(int kind, int nf, int mode, const ElementArgs& a, hipStream_t
       if (phys == FECX_J2_SURROGATE) { launch_modes<FECX_HEX8, 3, FECX_J2_SURROGATE>(mode, a, s); return 0; }
       if (phys == FECX_NH_SURROGATE) { launch_modes<FECX_HEX8, 3, FECX_NH_SURROGATE>(mode, a, s); return 0; }
     } else {
-      if (phys == FECX_LINEAR_ELASTIC) { launch_modes<FECX_TET10, 3, FECX_LINEAR_ELASTIC>(mode, a, s); return 0; }
-      if (phys == FECX_NEOHOOKEAN) { launch_modes<FECX_TET10, 3, FECX_NEOHOOKEAN>(mode, a, s); return 0; }
-      if (phys == FECX_J2) { launch_modes<FECX_TET10, 3, FECX_J2>(mode, a, s); return 0; }
-      if (phys == FECX_J2_SURROGATE) { launch_modes<FECX_TET10, 3, FECX_J2_SURROGATE>(mode, a, s); return 0; }
-      if (phys == FECX_NH_SURROGATE) { launch_modes<FECX_TET10, 3, FECX_NH_SURROGATE>(mode, a, s); return 0; }
+      if (phys == FECX_LINEAR_ELASTIC) { launch_tet10<3, FECX_LINEAR_ELASTIC>(mode, a, s); return 0; }
+      if (phys == FECX_NEOHOOKEAN) { launch_tet10<3, FECX_NEOHOOKEAN>(mode, a, s); return 0; }
+      if (phys == FECX_J2) { launch_tet10<3, FECX_J2>(mode, a, s); return 0; }
+      if (phys == FECX_J2_SURROGATE) { launch_tet10<3, FECX_J2_SURROGATE>(mode, a, s); return 0; }
+      if (phys == FECX_NH_SURROGATE) { launch_tet10<3, FECX_NH_SURROGATE>(mode, a, s); return 0; }
     }
   }
   return -1;

@@ -75,6 +75,7 @@ fecx::ElementArgs element_args(fecx_ctx* ctx, const double* U, const double* V) 
   a.Kel = get<float>(ctx, "Kel");
   a.flag = get<int>(ctx, "flag");
   a.mp = ctx->mp;
+  a.fp64_geometry = ctx->fp64_geometry ? 1 : 0;
   return a;
 }
 
@@ -939,6 +940,10 @@ int fecx_set_chunks(fecx_ctx* ctx, int chunks) {
 
 int fecx_set_merged_rows(fecx_ctx* ctx, int on) {
   return guard(ctx, [&] { ctx->merged_rows = on != 0; });
+}
+
+int fecx_set_fp64_geometry(fecx_ctx* ctx, int on) {
+  return guard(ctx, [&] { ctx->fp64_geometry = on != 0; });
 }
 
 int fecx_set_profiling(fecx_ctx* ctx, int on) {

@@ -140,6 +140,7 @@ struct ElementArgs {
   float* hq;                // MODE_QPGRAD output: ne x nq x (nf*nd) gradients, element storage order
   const float* qpmat;       // surrogate materials: ne x nq x (nf*nd + NC) stress and tangent data
   MatParams mp;
+  int fp64_geometry;        // tet10: element geometry, field gradient and stress in FP64 (fecx_set_fp64_geometry)
 };
 
 // returns 0, or -1 if the (kind, nf, phys) combination has no kernel
@@ -265,6 +266,7 @@ struct fecx_ctx {
   // pipelined assembly: element chunks on stream2, reduction chunks on the context's stream
   int chunks = 0;  // 0: chosen from the element storage size
   bool merged_rows = true;  // tet10 3-field reduction: both row buckets in one launch (fecx_set_merged_rows)
+  bool fp64_geometry = false;  // tet10 element geometry, field gradient and stress in FP64 (fecx_set_fp64_geometry)
   hipStream_t stream2 = nullptr;
   std::vector<hipEvent_t> events;
   // numeric error flag: device word set by the kernels, read back through a pinned host word after every assembly;

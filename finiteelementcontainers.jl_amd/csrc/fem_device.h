@@ -3,6 +3,8 @@
 // material models with their compact tangents.
 #pragma once
 
+#include <type_traits>
+
 #include "fecx_internal.h"
 
 namespace fecx {
@@ -26,11 +28,11 @@ __host__ __device__ constexpr int lane_cnt(int nnpe, int a) {
   return a >= nnpe ? 0 : (nnpe - 1) / 2 + 1 + ((nnpe % 2 == 0 && a < nnpe / 2) ? 1 : 0);
 }
 
-// ------------------------------------------------------------------ reference elements (FP32)
-template <int KIND>
-__device__ __forceinline__ float qweight() {
-  if constexpr (KIND == FECX_TET10) return 1.0f / 24.0f;
-  else return 1.0f;
+// ------------------------------------------------------------------ reference elements (FP32; FP64 forms for tet10)
+template <int KIND, class R = float>
+__device__ __forceinline__ R qweight() {
+  if constexpr (KIND == FECX_TET10) return R(1) / R(24);
+  else return R(1);
 }
 
 // Quadrature point q in reference coordinates: Gauss-Legendre 2 points per direction for quad4/hex8 (xi fastest),
@@ -45,6 +47,36 @@ __device__ __forceinline__ void qpoint(int q, float* xi) {
   } else {
     constexpr float qa = 0.13819660112501051f, qb = 0.58541019662496845f;
     xi[0] = q == 1 ? qb : qa; xi[1] = q == 2 ? qb : qa; xi[2] = q == 3 ? qb : qa;
+  }
+}
+// FP64 form (the tet10 element geometry is evaluated in FP64)
+template <int KIND>
+__device__ __forceinline__ void qpoint(int q, double* xi) {
+  static_assert(KIND == FECX_TET10, "FP64 quadrature points for tet10 only");
+  constexpr double qa = 0.13819660112501051, qb = 0.58541019662496845;
+  xi[0] = q == 1 ? qb : qa; xi[1] = q == 2 ? qb : qa; xi[2] = q == 3 ? qb : qa;
+}
+
+// tet10 N_a and dN_a/dxi in scalar type R
+template <class R>
+__device__ __forceinline__ void shape_tet10(int a, const R* xi, R& N, R* dN) {
+  // barycentric L0 = 1 - x - y - z, L1 = x, L2 = y, L3 = z
+  auto L = [&](int v) { return v == 0 ? R(1) - xi[0] - xi[1] - xi[2] : xi[v - 1]; };
+  auto dL = [&](int v, int j) { return v == 0 ? R(-1) : (v - 1 == j ? R(1) : R(0)); };
+  if (a < 4) {
+    const R l = L(a);
+    N = l * (2 * l - 1);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dN[j] = (4 * l - 1) * dL(a, j);
+  } else {
+    // edges 4:(0,1) 5:(1,2) 6:(2,0) 7:(0,3) 8:(1,3) 9:(2,3)
+    const int m = a - 4;
+    const int p = m < 3 ? m : m - 3;
+    const int r = m < 3 ? (m + 1) % 3 : 3;
+    const R lp = L(p), lr = L(r);
+    N = 4 * lp * lr;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dN[j] = 4 * (dL(p, j) * lr + lp * dL(r, j));
   }
 }
 
@@ -67,31 +99,21 @@ __device__ __forceinline__ void shape_node(int a, const float* xi, float& N, flo
     dN[1] = 0.125f * fx * sy * fz;
     dN[2] = 0.125f * fx * fy * sz;
   } else {
-    // barycentric L0 = 1 - x - y - z, L1 = x, L2 = y, L3 = z
-    auto L = [&](int v) { return v == 0 ? 1.0f - xi[0] - xi[1] - xi[2] : xi[v - 1]; };
-    auto dL = [&](int v, int j) { return v == 0 ? -1.0f : (v - 1 == j ? 1.0f : 0.0f); };
-    if (a < 4) {
-      const float l = L(a);
-      N = l * (2 * l - 1);
-#pragma unroll
-      for (int j = 0; j < 3; ++j) dN[j] = (4 * l - 1) * dL(a, j);
-    } else {
-      // edges 4:(0,1) 5:(1,2) 6:(2,0) 7:(0,3) 8:(1,3) 9:(2,3)
-      const int m = a - 4;
-      const int p = m < 3 ? m : m - 3;
-      const int r = m < 3 ? (m + 1) % 3 : 3;
-      const float lp = L(p), lr = L(r);
-      N = 4 * lp * lr;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) dN[j] = 4 * (dL(p, j) * lr + lp * dL(r, j));
-    }
+    shape_tet10<float>(a, xi, N, dN);
   }
+}
+
+template <int KIND>
+__device__ __forceinline__ void shape_node(int a, const double* xi, double& N, double* dN) {
+  static_assert(KIND == FECX_TET10, "FP64 shape functions for tet10 only");
+  shape_tet10<double>(a, xi, N, dN);
 }
 
 // ------------------------------------------------------------------ materials (FP32)
 // Each material provides:
 //   NC                         floats of compact tangent data per quadrature point
-//   eval(mp, H, so, sn, P, C)  stress P (NF x ND) and tangent data C from the field gradient H (and state)
+//   eval(mp, H, so, sn, P, C)  stress P (NF x ND) and tangent data C from the field gradient H (and state); H and P
+//                              in FP32 or (the tet10 element chain) FP64, C always FP32
 //   prep(mp, C, g, v)          per-node vector v (NV floats, v[0..ND) = g) from the shape gradient g: K_ab depends
 //                              on the pair only through v_a, v_b
 //   NA, side(mp, C, w, va, s)  the a-side factors s (NA floats) of one quadrature point, weight folded in
@@ -106,8 +128,8 @@ template <>
 struct Mat<FECX_POISSON> {
   static constexpr bool FROM_QP = false;  // evaluated in the element kernel
   static constexpr int NF = 1, NC = 1, NV = 3, NS = 0, S0 = 0;  // side scalars: C[S0 .. S0+NS)
-  template <int ND>
-  __device__ static void eval(const MatParams&, const float* H, const float*, float*, float* P, float*, bool&) {
+  template <int ND, class R>
+  __device__ static void eval(const MatParams&, const R* H, const float*, float*, R* P, float*, bool&) {
 #pragma unroll
     for (int j = 0; j < ND; ++j) P[j] = H[j];
   }
@@ -147,16 +169,16 @@ struct Mat<FECX_LINEAR_ELASTIC> {
   // components (extract_stress / extract_stiffness, Formulations.jl:391-419) are the same formulas over ND indices
   static constexpr bool FROM_QP = false;  // evaluated in the element kernel
   static constexpr int NF = 3, NC = 1, NV = 3, NS = 0, S0 = 0;
-  template <int ND>
-  __device__ static void eval(const MatParams& mp, const float* H, const float*, float*, float* P, float*, bool&) {
-    const float K = mp.p[1], G = mp.p[2], lm = K - 2.0f * G / 3.0f;
-    float tr = 0.0f;
+  template <int ND, class R>
+  __device__ static void eval(const MatParams& mp, const R* H, const float*, float*, R* P, float*, bool&) {
+    const R K = mp.p[1], G = mp.p[2], lm = K - R(2) * G / R(3);
+    R tr = R(0);
 #pragma unroll
     for (int i = 0; i < ND; ++i) tr += H[i * ND + i];
 #pragma unroll
     for (int i = 0; i < ND; ++i)
 #pragma unroll
-      for (int j = 0; j < ND; ++j) P[i * ND + j] = G * (H[i * ND + j] + H[j * ND + i]) + (i == j ? lm * tr : 0.0f);
+      for (int j = 0; j < ND; ++j) P[i * ND + j] = G * (H[i * ND + j] + H[j * ND + i]) + (i == j ? lm * tr : R(0));
   }
   template <int ND>
   __device__ static void prep(const MatParams&, const float*, const float* g, float* v) {
@@ -228,20 +250,22 @@ template <>
 struct Mat<FECX_NEOHOOKEAN> {
   static constexpr bool FROM_QP = false;  // evaluated in the element kernel
   static constexpr int NF = 3, NC = 10, NV = 6, NS = 1, S0 = 9;
-  template <int ND>
-  __device__ static void eval(const MatParams& mp, const float* H, const float*, float*, float* P, float* C, bool& bad) {
-    const float lam = mp.p[0], mu = mp.p[1];
-    const float tr = H[0] + H[4] + H[8];
-    const float trH2 = H[0] * H[0] + H[4] * H[4] + H[8] * H[8] + 2.0f * (H[1] * H[3] + H[2] * H[6] + H[5] * H[7]);
-    const float detH = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
-                       H[2] * (H[3] * H[7] - H[4] * H[6]);
-    const float Jm1 = tr + 0.5f * (tr * tr - trH2) + detH;
-    const float J = 1.0f + Jm1;
-    if (!(J > 0.0f)) bad = true;
-    const float lnJ = log1pf(Jm1);
-    const float F[9] = {1 + H[0], H[1], H[2], H[3], 1 + H[4], H[5], H[6], H[7], 1 + H[8]};
-    const float iJ = 1.0f / J;
-    float FiT[9];  // cofactor(F) / J
+  template <int ND, class R>
+  __device__ static void eval(const MatParams& mp, const R* H, const float*, float*, R* P, float* C, bool& bad) {
+    const R lam = mp.p[0], mu = mp.p[1];
+    const R tr = H[0] + H[4] + H[8];
+    const R trH2 = H[0] * H[0] + H[4] * H[4] + H[8] * H[8] + R(2) * (H[1] * H[3] + H[2] * H[6] + H[5] * H[7]);
+    const R detH = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
+                   H[2] * (H[3] * H[7] - H[4] * H[6]);
+    const R Jm1 = tr + R(0.5) * (tr * tr - trH2) + detH;
+    const R J = R(1) + Jm1;
+    if (!(J > R(0))) bad = true;
+    R lnJ;
+    if constexpr (std::is_same_v<R, float>) lnJ = log1pf(Jm1);
+    else lnJ = log1p(Jm1);
+    const R F[9] = {1 + H[0], H[1], H[2], H[3], 1 + H[4], H[5], H[6], H[7], 1 + H[8]};
+    const R iJ = R(1) / J;
+    R FiT[9];  // cofactor(F) / J
     FiT[0] = (F[4] * F[8] - F[5] * F[7]) * iJ; FiT[1] = (F[5] * F[6] - F[3] * F[8]) * iJ;
     FiT[2] = (F[3] * F[7] - F[4] * F[6]) * iJ; FiT[3] = (F[2] * F[7] - F[1] * F[8]) * iJ;
     FiT[4] = (F[0] * F[8] - F[2] * F[6]) * iJ; FiT[5] = (F[1] * F[6] - F[0] * F[7]) * iJ;
@@ -252,14 +276,14 @@ struct Mat<FECX_NEOHOOKEAN> {
       for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          const float htf = H[0 * 3 + i] * FiT[0 * 3 + j] + H[1 * 3 + i] * FiT[1 * 3 + j] + H[2 * 3 + i] * FiT[2 * 3 + j];
+          const R htf = H[0 * 3 + i] * FiT[0 * 3 + j] + H[1 * 3 + i] * FiT[1 * 3 + j] + H[2 * 3 + i] * FiT[2 * 3 + j];
           P[i * 3 + j] = mu * (H[i * 3 + j] + htf) + lam * lnJ * FiT[i * 3 + j];
         }
     }
     if (C) {
 #pragma unroll
-      for (int i = 0; i < 9; ++i) C[i] = FiT[i];
-      C[9] = mu - lam * lnJ;
+      for (int i = 0; i < 9; ++i) C[i] = (float)FiT[i];
+      C[9] = (float)(mu - lam * lnJ);
     }
   }
   template <int ND>
@@ -380,9 +404,18 @@ template <>
 struct Mat<FECX_J2> {
   static constexpr bool FROM_QP = false;  // evaluated in the element kernel
   static constexpr int NF = 3, NC = 9, NV = 6, NS = 3, S0 = 6;
-  template <int ND>
-  __device__ static void eval(const MatParams& mp, const float* H, const float* so, float* sn, float* P, float* C, bool&) {
-    j2_update(mp.p, H, so, sn, P, C);
+  template <int ND, class R>
+  __device__ static void eval(const MatParams& mp, const R* H, const float* so, float* sn, R* P, float* C, bool&) {
+    if constexpr (std::is_same_v<R, float>) {
+      j2_update(mp.p, H, so, sn, P, C);
+    } else {  // the FP64 element chain: the return mapping itself stays FP32
+      float Hf[9], Pf[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) Hf[c] = (float)H[c];
+      j2_update(mp.p, Hf, so, sn, Pf, C);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) P[c] = Pf[c];
+    }
   }
   template <int ND>
   __device__ static void prep(const MatParams&, const float* C, const float* g, float* v) {

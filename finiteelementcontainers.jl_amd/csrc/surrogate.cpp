@@ -412,6 +412,7 @@ int fecx_qp_gradients(fecx_ctx* ctx, const double* U, float* Hq) {
     ea.U = U;
     ea.flag = get<int>(ctx, "flag");
     ea.mp = ctx->mp;
+    ea.fp64_geometry = ctx->fp64_geometry ? 1 : 0;
     ea.hq = (float*)dev_atleast(ctx, "hq", sizeof(float) * nqp * 9);
     HIPCHK(hipMemsetAsync(ea.flag, 0, sizeof(int), ctx->stream));
     ctx->flag_dirty = true;  // not read back here: the next assembly clears it first

@@ -352,6 +352,15 @@ int fecx_set_chunks(fecx_ctx* ctx, int chunks);
  * 0 runs the two row buckets as two launches. Results do not depend on it (same sums, same order). */
 int fecx_set_merged_rows(fecx_ctx* ctx, int on);
 
+/* Precision of the tet10 element chain (the reference evaluates its element loop in FP64 throughout,
+ * src/assemblers/Assemblers.jl:402-432): 0 (the default) evaluates every element quantity in FP32; 1 evaluates the
+ * isoparametric map, shape gradients, field gradient and stress of tet10 elements in FP64, from FP64 coordinate and
+ * field differences, and passes their FP32 roundings to the tangent, the residual contraction and the element
+ * outputs, which stay FP32 (the reduction sums them in FP64 as always). On jittered tet10 meshes the residual's
+ * interior rows are cancelling sums of O(1) element contributions, so the FP32 error grows like eps / h; this keeps
+ * every output within 1e-6 of FP64 at 80^3 x 6 elements. Other element kinds are unaffected. */
+int fecx_set_fp64_geometry(fecx_ctx* ctx, int on);
+
 /* Scatter design of matrix assemblies (no counterpart in the reference, whose element loop stores element-private
  * matrices and sparse! sums them afterwards, src/assemblers/Matrix.jl:35-75, SparsityPatterns.jl:301-329):
  *   FECX_SCATTER_AUTO       (default) FUSED where it is the faster design on MI355X (vector-valued materials, nf = 3),

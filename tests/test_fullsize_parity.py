@@ -9,7 +9,11 @@
       lands above 1e-6 (measured 2.6e-6 for the residual's 2-norm, 1.2-1.6e-6 in the max-norms): interior residuals
       are cancelling sums of O(1) element contributions, so their relative error grows like eps / h, and on the
       jittered tets it comes from every stage of the element chain (geometry, stress, contraction) alike. The test
-      states that error: 4e-6 in the 2-norm, 3e-6 in the max-norm."""
+      states that error: 4e-6 in the 2-norm, 3e-6 in the max-norm.
+  neohookean_tet10 80^3 x 6 with the FP64 element geometry, field gradient and stress (fecx_set_fp64_geometry):
+      residual, CSR values and both actions, each within 1e-6 in both norms."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -89,3 +93,34 @@ def test_neohookean_tet10_80_residual_and_actions(fec, orc, gpu):
           f"ASM {eAM[0]:.2e}/{eAM[1]:.2e}", flush=True)
     for e in (eR, eMF, eAM):
         assert e[0] < 4e-6 and e[1] < 3e-6
+
+
+def test_neohookean_tet10_80_fp64_geometry_every_output_within_1e6(fec, orc, gpu):
+    print("[fullsize parity] neohookean_tet10 80^3 x 6, FP64 element geometry", flush=True)
+    wl = fec.workloads.neohookean_tet10(80)
+    m = wl.mesh
+    asm, p = _setup(fec, wl)
+    asm.ctx.call("fecx_set_fp64_geometry", ctypes.c_int(1))
+    Uu = torch.from_numpy(wl.U).to(gpu)
+    fec.assemble_vector_and_stiffness_(asm, Uu, p)
+    R = fec.residual(asm).cpu().numpy().copy()
+    K = asm.stiffness_storage.cpu().numpy()
+    props = wl.physics.props()
+    orc.set_threads(16)
+    R_ref = orc.assemble_residual(orc.TET10, m.coords, m.conn, 3, orc.NEOHOOKEAN, props, wl.U)
+    eR = _errs(R, R_ref)
+    del R_ref
+    N = 3 * m.num_nodes
+    _, d2u = orc.update_dofs(N, np.zeros(0, np.int64))
+    st = orc.assemble_stiffness_storage(orc.TET10, m.coords, m.conn, 3, orc.NEOHOOKEAN, props, wl.U)
+    rp, cv, vals = orc.sparse_csr_rows(3, m.conn, d2u, N, st)
+    del st
+    rp_g, cv_g = (t.cpu().numpy() + 1 for t in asm.pattern())
+    assert np.array_equal(rp_g, rp) and np.array_equal(cv_g, cv)
+    eK = _errs(K, vals)
+    del K, vals, rp, cv, rp_g, cv_g
+    eMF, eAM = _actions(fec, orc, wl, asm, p, Uu, orc.TET10)
+    print(f"[fullsize parity] tet10 80^3x6 fp64 geometry (l2/max): R {eR[0]:.2e}/{eR[1]:.2e} "
+          f"K {eK[0]:.2e}/{eK[1]:.2e} MF {eMF[0]:.2e}/{eMF[1]:.2e} ASM {eAM[0]:.2e}/{eAM[1]:.2e}", flush=True)
+    for e in (eR, eK, eMF, eAM):
+        assert e[0] < 1e-6 and e[1] < 1e-6

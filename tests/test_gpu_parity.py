@@ -77,6 +77,9 @@ def _workload(fec, name):
         return W.neohookean_hex8(6)
     if name == "neohookean_tet10":
         return W.neohookean_tet10(3)
+    if name == "mechanics_tet10":
+        wl = W.neohookean_tet10(3)
+        return W.Workload("mechanics_tet10", wl.mesh, fec.Mechanics(), 3, 0.01 * wl.U)
     if name == "j2_hex8":
         return W.j2_hex8(4, dirichlet=True)
     raise KeyError(name)
@@ -84,9 +87,22 @@ def _workload(fec, name):
 
 @pytest.mark.parametrize("name", CASES)
 def test_operators_match_oracle(fec, orc, gpu, name):
-    print(f"[gpu parity] {name}", flush=True)
+    _operators_vs_oracle(fec, orc, gpu, name, fp64_geometry=False)
+
+
+@pytest.mark.parametrize("name", ["poisson_tet10", "mechanics_tet10", "neohookean_tet10"])
+def test_operators_match_oracle_fp64_geometry(fec, orc, gpu, name):
+    """tet10 with the FP64 element geometry, field gradient and stress (fecx_set_fp64_geometry): the same outputs
+    against the oracle, the same bounds, and the same bit-for-bit agreement of fused and separate calls."""
+    _operators_vs_oracle(fec, orc, gpu, name, fp64_geometry=True)
+
+
+def _operators_vs_oracle(fec, orc, gpu, name, fp64_geometry):
+    print(f"[gpu parity] {name}" + (" fp64 geometry" if fp64_geometry else ""), flush=True)
     wl = _workload(fec, name)
     asm, p = _setup(fec, wl, gpu)
+    if fp64_geometry:
+        asm.ctx.call("fecx_set_fp64_geometry", ctypes.c_int(1))
     unknown = asm.dof.unknown_dofs
     N = wl.nf * wl.mesh.num_nodes
     _, d2u = orc.update_dofs(N, asm.dof.dirichlet_dofs)
@@ -120,7 +136,8 @@ def test_operators_match_oracle(fec, orc, gpu, name):
 
     assert np.array_equal(K.rowptr.cpu().numpy() + 1, rp) and np.array_equal(K.colval.cpu().numpy() + 1, cv)
     errs = {"R": rel(R, R_ref), "K": rel(Kvals, vals), "KVa": rel(KVa, KVa_ref), "KVm": rel(KVm, KVm_ref)}
-    print(f"[gpu parity] {name}: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    print(f"[gpu parity] {name}{' fp64 geometry' if fp64_geometry else ''}: " +
+          " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
     assert errs["R"] < TOL["R"] and errs["K"] < TOL["K"]
     assert errs["KVa"] < TOL["KV"] and errs["KVm"] < TOL["KV"]
 
