@@ -296,10 +296,11 @@ class SparseMatrixAssembler:
     """SparseMatrixAssembler (src/assemblers/SparseMatrixAssembler.jl:7-133) with one libfecx context.
 
     sparse_matrix_type "csr" or "csc" (the reference's default, SparseMatrixAssembler.jl:66); use_condensed keeps every
-    DOF in the pattern and adjusts the results for the constraints in the accessors (Utils.jl:53-167)."""
+    DOF in the pattern and adjusts the results for the constraints in the accessors (Utils.jl:53-167); fp64_geometry
+    evaluates the tet10 element geometry, field gradient and stress in FP64 (fecx_set_fp64_geometry)."""
 
     def __init__(self, var, sparse_matrix_type="csr", device=0, matrix_free=False, use_condensed=False,
-                 penalty_scale=1.0e6):
+                 penalty_scale=1.0e6, fp64_geometry=False):
         if sparse_matrix_type not in ("csr", "csc"):
             raise FECXError(-4, f"Unsupported sparse matrix type {sparse_matrix_type}. :csr and :csc are assembled "
                                 "on the GPU.")
@@ -318,6 +319,8 @@ class SparseMatrixAssembler:
         _set_mesh(self.ctx, mesh, var.fspace.elem_kind)
         if self.use_condensed:
             self.ctx.call("fecx_set_condensed", ctypes.c_int(1))
+        if fp64_geometry:
+            self.ctx.call("fecx_set_fp64_geometry", ctypes.c_int(1))
         self._pattern = None
         self._k_adjusted = self._m_adjusted = True
         self.update_dofs(np.zeros(0, dtype=np.int64))

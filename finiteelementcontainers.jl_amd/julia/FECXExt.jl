@@ -111,13 +111,14 @@ function elem_kind(fspace, b)
 end
 
 """
-    FECXAssembler(var; sparse_matrix_type = :csr, use_condensed = false, device = 0)
+    FECXAssembler(var; sparse_matrix_type = :csr, use_condensed = false, device = 0, fp64_geometry = false)
 
 Drop-in for `SparseMatrixAssembler(var; sparse_matrix_type, use_condensed)` (SparseMatrixAssembler.jl:59-133)
-whose element loops and sparse reduction run in libfecx.so on an MI355X.
+whose element loops and sparse reduction run in libfecx.so on an MI355X. `fp64_geometry = true` evaluates the tet10
+element geometry, field gradient and stress in FP64 (fecx_set_fp64_geometry).
 """
 function FECXAssembler(var::FEC.AbstractFunction; sparse_matrix_type::Symbol = :csr, use_condensed::Bool = false,
-                       device::Integer = 0)
+                       device::Integer = 0, fp64_geometry::Bool = false)
   sparse_matrix_type in (:csr, :csc) || throw(FEC.UnsupportedSparseMatrixType(sparse_matrix_type))
   dof = FEC.DofManager(var; use_condensed = use_condensed)
   fspace = FEC.function_space(dof)
@@ -134,6 +135,7 @@ function FECXAssembler(var::FEC.AbstractFunction; sparse_matrix_type::Symbol = :
                    (Ptr{Cvoid}, Cint, Cint, Int64, Ptr{Float64}, Cint, Ptr{Int64}, Ptr{Int64}),
                    ctx.ptr, kind, size(X, 1), size(X, 2), X, nb, block_ne, conn))
   use_condensed && check(ctx, ccall((:fecx_set_condensed, libfecx), Cint, (Ptr{Cvoid}, Cint), ctx.ptr, 1))
+  fp64_geometry && check(ctx, ccall((:fecx_set_fp64_geometry, libfecx), Cint, (Ptr{Cvoid}, Cint), ctx.ptr, 1))
   ND, NN = size(dof)
   asm = FECXAssembler{use_condensed, typeof(var)}(
     dof, ctx, sparse_matrix_type, Int64[], Int64[], AMDGPU.zeros(Float64, ND * NN),

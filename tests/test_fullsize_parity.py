@@ -12,8 +12,6 @@
       states that error: 4e-6 in the 2-norm, 3e-6 in the max-norm.
   neohookean_tet10 80^3 x 6 with the FP64 element geometry, field gradient and stress (fecx_set_fp64_geometry):
       residual, CSR values and both actions, each within 1e-6 in both norms."""
-import ctypes
-
 import numpy as np
 import pytest
 import torch
@@ -25,9 +23,10 @@ def _errs(a, b):
     return (float(np.linalg.norm(a - b) / np.linalg.norm(b)), float(np.abs(a - b).max() / np.abs(b).max()))
 
 
-def _setup(fec, wl):
+def _setup(fec, wl, fp64_geometry=False):
     V = fec.FunctionSpace(wl.mesh, "H1Field", "Lagrange")
-    asm = fec.SparseMatrixAssembler(fec.VectorFunction(V, "displ"), sparse_matrix_type="csr", device=0)
+    asm = fec.SparseMatrixAssembler(fec.VectorFunction(V, "displ"), sparse_matrix_type="csr", device=0,
+                                    fp64_geometry=fp64_geometry)
     p = fec.create_parameters(wl.mesh, asm, wl.physics)
     return asm, p
 
@@ -99,8 +98,7 @@ def test_neohookean_tet10_80_fp64_geometry_every_output_within_1e6(fec, orc, gpu
     print("[fullsize parity] neohookean_tet10 80^3 x 6, FP64 element geometry", flush=True)
     wl = fec.workloads.neohookean_tet10(80)
     m = wl.mesh
-    asm, p = _setup(fec, wl)
-    asm.ctx.call("fecx_set_fp64_geometry", ctypes.c_int(1))
+    asm, p = _setup(fec, wl, fp64_geometry=True)
     Uu = torch.from_numpy(wl.U).to(gpu)
     fec.assemble_vector_and_stiffness_(asm, Uu, p)
     R = fec.residual(asm).cpu().numpy().copy()
