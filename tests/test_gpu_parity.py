@@ -145,6 +145,15 @@ def _operators_vs_oracle(fec, orc, gpu, name, fp64_geometry):
     fec.assemble_vector_and_stiffness_(asm, Uu, p)
     assert np.array_equal(fec.residual(asm).cpu().numpy(), R)
     assert np.array_equal(asm.stiffness_storage.cpu().numpy(), Kvals)
+    if fp64_geometry:  # the allocation-free entry point runs the same element kernel
+        P = fec.assembler._ptr
+        R2 = torch.full_like(asm.residual_storage, float("nan"))
+        K2 = torch.full_like(asm.stiffness_storage, float("nan"))
+        asm.ctx.call("fecx_assemble_vector_matrix_noalloc", P(p.field), P(R2), P(K2))
+        R1 = torch.full_like(R2, float("nan"))
+        K1 = torch.full_like(K2, float("nan"))
+        asm.ctx.call("fecx_assemble_vector_matrix", P(p.field), P(R1), P(K1))
+        assert torch.equal(R1, R2) and torch.equal(K1, K2)
 
     if st_old is not None:
         ns = np.zeros(st_old.size, dtype=np.float32)
