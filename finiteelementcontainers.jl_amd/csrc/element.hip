@@ -29,9 +29,15 @@ struct NodeDiffs {
 template <class RT, int NNPE, int ND, int NF>
 struct NodeDiffs<RT, NNPE, ND, NF, true> {};
 
+// tet10 modes that stage element blocks keep the node differences in the work area (see GW below)
+template <int KIND, int MODE>
+constexpr bool diffs_in_work() {
+  return KIND == FECX_TET10 && (MODE & (MODE_K | MODE_ACTION_ASM)) != 0;
+}
+
 template <int KIND, int NF, int PHYS, int MODE, bool G64 = false>
 struct ElemLDS : NodeDiffs<std::conditional_t<G64, double, float>, ET<KIND>::NNPE, ET<KIND>::ND, NF,
-                           KIND == FECX_TET10> {
+                           diffs_in_work<KIND, MODE>()> {
   using T = ET<KIND>;
   static constexpr bool DO_R = (MODE & MODE_R) != 0, DO_K = (MODE & MODE_K) != 0;
   static constexpr bool DO_MF = (MODE & MODE_ACTION_MF) != 0, DO_ASM = (MODE & MODE_ACTION_ASM) != 0;
@@ -56,11 +62,11 @@ struct ElemLDS : NodeDiffs<std::conditional_t<G64, double, float>, ET<KIND>::NNP
   static constexpr bool NOQP = HEX && !(DO_K || DO_ASM || DO_D);  // no node-phase consumer of quadrature data
   // G64 (tet10, fecx_set_fp64_geometry): the element geometry, field gradient and stress are evaluated in FP64 (from
   // FP64 coordinate and field differences); their FP32 roundings feed the tangent, the residual contraction and the
-  // stored rows. tet10: the differences (Geo) live in the wavefront's work area beside the quadrature data (GW), in
-  // space that the element blocks take only once both are consumed.
+  // stored rows. tet10 modes that stage element blocks: the differences (Geo) live in the wavefront's work area beside
+  // the quadrature data (GW), in space that the element blocks take only once both are consumed.
   static_assert(!G64 || KIND == FECX_TET10, "FP64 element geometry for tet10 only");
   using RT = std::conditional_t<G64, double, float>;
-  static constexpr bool GW = KIND == FECX_TET10;
+  static constexpr bool GW = diffs_in_work<KIND, MODE>();
   using Geo = NodeDiffs<RT, T::NNPE, T::ND, NF, false>;
   float x0[T::ND];
   float v[(DO_MF || DO_ASM) ? T::NNPE : 1][NF];
