@@ -69,6 +69,16 @@ def test_library_exports_every_header_symbol(fec):
     assert set(fec.exported_symbols()) <= declared
 
 
+def test_julia_shim_binds_declared_symbols():
+    """Every entry point the Julia package-extension source calls through ccall is declared in the header."""
+    hdr = open(os.path.join(ROOT, "include", "fecx.h")).read()
+    declared = set(re.findall(r"\b(fecx_\w+)\s*\(", hdr))
+    jl = open(os.path.join(ROOT, "finiteelementcontainers.jl_amd", "julia", "FECXExt.jl")).read()
+    bound = set(re.findall(r"ccall\(\(:(fecx_\w+),", jl))
+    assert bound, "no ccall bindings parsed"
+    assert bound <= declared, sorted(bound - declared)
+
+
 def test_host_only_context_refuses_kernels(fec):
     mesh = fec.StructuredMesh("hex", (0, 0, 0), (1, 1, 1), (3, 3, 3))
     hp = fec.HostPattern(mesh, 1)
