@@ -423,6 +423,13 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
     if constexpr (!SC && !NOQP) {
       if (r == 0) Q.w = w;
     }
+    if constexpr (!(DO_K || DO_ASM || DO_D) && !SC && !NOQP) {  // modes without element blocks: g_a as it is, stored
+#pragma unroll                                                  // before the material so that it need not stay live
+      for (int k = 0; k < NMY; ++k)
+        if (mine(k))
+#pragma unroll
+          for (int j = 0; j < ND; ++j) Q.v[r + S * k][j] = g[k][j];
+    }
     if constexpr (DO_G) {  // quadrature-point gradients only, element storage order
       if (r == 0) {
         float* o = args.hq + ((int64_t)e * NQ + q) * D;
@@ -538,12 +545,6 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
             for (int c = 0; c < M::NV; ++c) Q.v[r + S * k][c] = va[c];
           }
       }
-    } else if constexpr (!SC && !NOQP) {
-#pragma unroll
-      for (int k = 0; k < NMY; ++k)
-        if (mine(k))
-#pragma unroll
-          for (int j = 0; j < ND; ++j) Q.v[r + S * k][j] = g[k][j];
     }
     if constexpr (DO_MF) {
       float Gv[D] = {};
