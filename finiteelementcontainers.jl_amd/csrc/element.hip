@@ -184,10 +184,10 @@ struct PairJobs {
   static constexpr int NNPE = ET<KIND>::NNPE;
   static constexpr int LANES = KIND == FECX_TET10 ? 16 : NNPE;  // lanes that form blocks
   static constexpr int NP = KIND == FECX_TET10 ? 4 : ET<KIND>::MAXP;  // blocks per lane, at most
-  __device__ static int center(int lane) { return lane < NNPE ? lane : lane - NNPE; }
+  __host__ __device__ static constexpr int center(int lane) { return lane < NNPE ? lane : lane - NNPE; }
   // block p of `lane`: false if the lane has none, else the other node b, the storage slot and whether the slot holds
   // the transpose of the computed K_cb
-  __device__ static bool job(int lane, int p, int& b, int& slot, bool& tr) {
+  __host__ __device__ static constexpr bool job(int lane, int p, int& b, int& slot, bool& tr) {
     const int c = center(lane);
     if (KIND != FECX_TET10 || lane < NNPE) {
       const int lim = KIND == FECX_TET10 ? 4 : lane_cnt(NNPE, c);
@@ -206,6 +206,37 @@ struct PairJobs {
     return j == 0 ? c <= 5 : (j == 1 ? c <= 3 : (j == 2 ? c <= 4 : false));
   }
 };
+
+// tet10 block addressing, built at compile time from PairJobs, pair_ref and pair_pos, so that a lane loads its bytes
+// once instead of evaluating their index arithmetic (~110 instructions per wavefront in the staging alone):
+//   stage[lane]: byte p = job p of the lane's node phase: position of its block in the element record (bits 0-5),
+//                the block is stored transposed (bit 6), the job exists (bit 7);
+//   rows[a][b]:  position of the block of the pair {a, b} (bits 0-5), the block is K_ab as stored (bit 6).
+struct Tet10Addr {
+  uint32_t stage[16];
+  uint8_t rows[10][12];
+};
+constexpr Tet10Addr make_tet10_addr() {
+  Tet10Addr t{};
+  using PJ = PairJobs<FECX_TET10>;
+  for (int lane = 0; lane < 16; ++lane) {
+    uint32_t w = 0;
+    for (int p = 0; p < PJ::NP; ++p) {
+      int b = 0, slot = 0;
+      bool tr = false;
+      if (PJ::job(lane, p, b, slot, tr))
+        w |= (uint32_t)(pair_pos(10, PJ::center(lane), b, slot) | (tr ? 64 : 0) | 128) << (8 * p);
+    }
+    t.stage[lane] = w;
+  }
+  for (int a = 0; a < 10; ++a)
+    for (int b = 0; b < 10; ++b) {
+      const PairRef pr = pair_ref(10, a, b);
+      t.rows[a][b] = (uint8_t)(pair_pos(10, a, b, pr.slot) | (pr.ab ? 64 : 0));
+    }
+  return t;
+}
+__constant__ Tet10Addr c_tet10_addr = make_tet10_addr();
 
 // One lane's gather inputs: its local node's coordinates (hi/lo FP32 split) and field values. Loaded apart from the
 // element computation so that a caller can issue the loads early (the fused kernel fetches the next layer's while
@@ -590,6 +621,9 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
   using PJ = PairJobs<KIND>;
   constexpr int MAXP = PJ::NP;
   float kreg[(DO_K || DO_ASM) ? MAXP : 1][NF * NF];
+  // tet10: the lane's block-staging addresses, loaded here so that the load is long done when the blocks are staged
+  uint32_t stage_addr = 0;
+  if constexpr (KIND == FECX_TET10 && (DO_K || DO_ASM)) stage_addr = c_tet10_addr.stage[lane];
   if (valid && lane < NNPE) {
     const int a = lane;
     if constexpr ((DO_R || DO_MF) && !QR && !QM) {
@@ -683,10 +717,19 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
   auto stage_blocks = [&](float* Kblk) {
 #pragma unroll
     for (int p = 0; p < MAXP; ++p) {
-      int b, slot;
-      bool tr;
-      if (PJ::job(lane, p, b, slot, tr)) {
-        float* dst = Kblk + pair_pos(NNPE, PJ::center(lane), b, slot) * (NF * NF);
+      int b = 0, slot = 0, pos = 0;
+      bool tr = false, ok;
+      if constexpr (KIND == FECX_TET10) {
+        const uint32_t t = (stage_addr >> (8 * p)) & 0xFFu;
+        ok = (t & 128u) != 0;
+        tr = (t & 64u) != 0;
+        pos = (int)(t & 63u);
+      } else {
+        ok = PJ::job(lane, p, b, slot, tr);
+        pos = pair_pos(NNPE, PJ::center(lane), b, slot);
+      }
+      if (ok) {
+        float* dst = Kblk + pos * (NF * NF);
 #pragma unroll
         for (int i = 0; i < NF; ++i)
 #pragma unroll
@@ -723,13 +766,29 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
     phase_sync<WORK>();
     if (valid && lane < NNPE) {
       float kv[NF] = {};
+      uint32_t rows[3] = {0u, 0u, 0u};  // tet10: the lane's row of block addresses
+      if constexpr (KIND == FECX_TET10) {
+        const uint32_t* r = reinterpret_cast<const uint32_t*>(c_tet10_addr.rows[lane]);
+        rows[0] = r[0]; rows[1] = r[1]; rows[2] = r[2];
+      }
+#pragma unroll
       for (int b = 0; b < NNPE; ++b) {
-        const PairRef pr = pair_ref(NNPE, lane, b);
-        const float* blk = work.kb(grp) + pair_pos(NNPE, lane, b, pr.slot) * (NF * NF);
+        int pos;
+        bool ab;
+        if constexpr (KIND == FECX_TET10) {
+          const uint32_t t = (rows[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+          pos = (int)(t & 63u);
+          ab = (t & 64u) != 0;
+        } else {
+          const PairRef pr = pair_ref(NNPE, lane, b);
+          pos = pair_pos(NNPE, lane, b, pr.slot);
+          ab = pr.ab;
+        }
+        const float* blk = work.kb(grp) + pos * (NF * NF);
 #pragma unroll
         for (int i = 0; i < NF; ++i)
 #pragma unroll
-          for (int k = 0; k < NF; ++k) kv[i] = fmaf(pr.ab ? blk[i * NF + k] : blk[k * NF + i], L.v[b][k], kv[i]);
+          for (int k = 0; k < NF; ++k) kv[i] = fmaf(ab ? blk[i * NF + k] : blk[k * NF + i], L.v[b][k], kv[i]);
       }
 #pragma unroll
       for (int i = 0; i < NF; ++i) rel_out(lane * NF + i, kv[i]);

@@ -41,7 +41,7 @@ struct PairRef {
 __host__ __device__ constexpr int pair_lane_off(int nnpe, int a) {
   return a * ((nnpe - 1) / 2 + 1) + ((nnpe % 2 == 0) ? (a < nnpe / 2 ? a : nnpe / 2) : 0);
 }
-__host__ __device__ inline PairRef pair_ref(int nnpe, int a, int b) {
+__host__ __device__ constexpr PairRef pair_ref(int nnpe, int a, int b) {
   const int dmax = (nnpe - 1) / 2;
   const int d1 = (b - a + nnpe) % nnpe;
   if (d1 <= dmax) return {pair_lane_off(nnpe, a) + d1, true};
@@ -56,7 +56,7 @@ __host__ __device__ inline PairRef pair_ref(int nnpe, int a, int b) {
 // so that the rows of each node kind -- the reduction's two row buckets, launched one after the other -- read one
 // contiguous region of the record (blocks 0-44 and 21-54) instead of blocks spread over all of it. Other kinds: the
 // slot itself.
-__host__ __device__ inline int pair_pos(int nnpe, int a, int b, int slot) {
+__host__ __device__ constexpr int pair_pos(int nnpe, int a, int b, int slot) {
   if (nnpe != 10) return slot;
   const int lo = a < b ? a : b, hi = a < b ? b : a;
   if (lo >= 4) {
