@@ -931,9 +931,13 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
   const int tid = threadIdx.x, grp = tid / 8;
   const int sx = grp % SX, sy = grp / SX;  // this lane group's element slot
   const int nex = f.nx - 1, ney = f.ny - 1, nez = f.nz - 1;
-  const int64_t units = (int64_t)f.tiles_x * f.tiles_y * f.nz;
-  int64_t u = units * blockIdx.x / gridDim.x;
-  const int64_t u_end = units * (blockIdx.x + 1) / gridDim.x;
+  // this work-group's share [units b / G, units (b + 1) / G) of the units, in 32-bit arithmetic (a 64-bit division is
+  // ~150 instructions): units = q G + rr, so units b / G = q b + rr b / G, where rr b < G^2 (units < 2^31 and
+  // G < 2^16, both checked at launch)
+  const uint32_t units = (uint32_t)f.tiles_x * (uint32_t)f.tiles_y * (uint32_t)f.nz;
+  const uint32_t G = gridDim.x, uq = units / G, ur = units % G;
+  uint32_t u = uq * blockIdx.x + ur * blockIdx.x / G;
+  const uint32_t u_end = uq * (blockIdx.x + 1) + ur * (blockIdx.x + 1) / G;
   // reduction task of this lane: the blocks (dx, dy, *) of a node's row, or the node's residual row (the lanes of the
   // last wavefront, which also hand the plane's row ranges to the copy-out)
   const bool task = tid < TL::TASKS;
@@ -966,11 +970,11 @@ void fused_hex8_kernel(ElementArgs args, FusedArgs f) {
   };
 
   while (u < u_end) {
-    const int64_t tile = u / f.nz;
-    const int p0 = (int)(u % f.nz);
-    const int p1 = (int)(u_end - u < (int64_t)(f.nz - p0) ? p0 + (u_end - u) : f.nz);
+    const uint32_t tile = u / (uint32_t)f.nz;
+    const int p0 = (int)(u % (uint32_t)f.nz);
+    const int p1 = u_end - u < (uint32_t)(f.nz - p0) ? p0 + (int)(u_end - u) : f.nz;
     u += p1 - p0;
-    const int x0 = (int)(tile % f.tiles_x) * TL::TX, y0 = (int)(tile / f.tiles_x) * TL::TY;
+    const int x0 = (int)(tile % (uint32_t)f.tiles_x) * TL::TX, y0 = (int)(tile / (uint32_t)f.tiles_x) * TL::TY;
     const int ex = x0 - 1 + sx, ey = y0 - 1 + sy;
     const bool col_ok = ex >= 0 && ex < nex && ey >= 0 && ey < ney;
     const int gx = x0 + li, gy = y0 + lj;
@@ -1249,7 +1253,7 @@ int launch_scatter_hex8(int nf, int scatter, const ElementArgs& a, const FusedAr
 // One work-group per slot the device holds at once (the kernel's occupancy times the CU count), each with an equal
 // share of the (tile, plane) units.
 template <int NF, int PHYS>
-static void launch_fused(const ElementArgs& a, FusedArgs f, hipStream_t s) {
+static bool launch_fused(const ElementArgs& a, FusedArgs f, hipStream_t s) {
   using TL = FusedTile<FUSED_SX, FUSED_SY>;
   auto kern = fused_hex8_kernel<NF, PHYS, FUSED_SX, FUSED_SY>;
   static int slots = 0;  // per instantiation; the same on every gfx950 device
@@ -1264,16 +1268,19 @@ static void launch_fused(const ElementArgs& a, FusedArgs f, hipStream_t s) {
   f.tiles_y = (f.ny + TL::TY - 1) / TL::TY;
   const int64_t units = (int64_t)f.tiles_x * f.tiles_y * f.nz;
   const unsigned grid = (unsigned)(units < slots ? units : slots);
+  if (units >= (int64_t(1) << 31) || grid >= (1u << 16)) return false;  // the kernel's 32-bit work split
   kern<<<grid, TL::THREADS, 0, s>>>(a, f);
+  return true;
 }
 
 int launch_fused_hex8(int nf, const ElementArgs& a, const FusedArgs& f, hipStream_t s) {
   const int phys = a.mp.phys;
-  if (nf == 1 && phys == FECX_POISSON) { launch_fused<1, FECX_POISSON>(a, f, s); return 0; }
-  if (nf == 3 && phys == FECX_LINEAR_ELASTIC) { launch_fused<3, FECX_LINEAR_ELASTIC>(a, f, s); return 0; }
-  if (nf == 3 && phys == FECX_NEOHOOKEAN) { launch_fused<3, FECX_NEOHOOKEAN>(a, f, s); return 0; }
-  if (nf == 3 && phys == FECX_J2) { launch_fused<3, FECX_J2>(a, f, s); return 0; }
-  return -1;
+  bool ok = false;
+  if (nf == 1 && phys == FECX_POISSON) ok = launch_fused<1, FECX_POISSON>(a, f, s);
+  else if (nf == 3 && phys == FECX_LINEAR_ELASTIC) ok = launch_fused<3, FECX_LINEAR_ELASTIC>(a, f, s);
+  else if (nf == 3 && phys == FECX_NEOHOOKEAN) ok = launch_fused<3, FECX_NEOHOOKEAN>(a, f, s);
+  else if (nf == 3 && phys == FECX_J2) ok = launch_fused<3, FECX_J2>(a, f, s);
+  return ok ? 0 : -1;
 }
 
 // tet10 in FP32 or with the FP64 element geometry (ElementArgs::fp64_geometry)

@@ -155,7 +155,7 @@ struct FusedArgs {
   double* vals;                  // CSR values (or nullptr)
   double* R;                     // full NF x NN residual field (or nullptr)
 };
-// returns 0, or -1 if the (nf, phys) combination has no fused kernel
+// returns 0, or -1 if the (nf, phys) combination has no fused kernel (or the mesh exceeds 2^31 tile planes)
 int launch_fused_hex8(int nf, const ElementArgs& a, const FusedArgs& f, hipStream_t s);
 // the atomic (scatter 1) and coloured (scatter 2) designs on structured hex8 meshes; vals and R zero-filled first
 int launch_scatter_hex8(int nf, int scatter, const ElementArgs& a, const FusedArgs& f, hipStream_t s);
