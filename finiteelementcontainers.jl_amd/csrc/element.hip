@@ -145,9 +145,12 @@ __device__ __forceinline__ void phase_sync() {
   }
 }
 
+// A lane permutation within rows of 16 lanes (quad_perm, row_half_mirror: every lane has a source). bound_ctrl set,
+// which changes nothing for these permutations, lets the compiler fold the move into the add that consumes it
+// (v_add_f32_dpp).
 template <int CTRL>
 __device__ __forceinline__ float dpp_mov(float x) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov_f64(double x) {  // the same permutation of both 32-bit halves
@@ -372,8 +375,8 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
         v += dpp_mov_f64<0xB1>(v);  // quad_perm 1,0,3,2
         v += dpp_mov_f64<0x4E>(v);  // quad_perm 2,3,0,1
       } else {
-        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
-        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+        v += dpp_mov<0xB1>(v);  // quad_perm 1,0,3,2
+        v += dpp_mov<0x4E>(v);  // quad_perm 2,3,0,1
       }
     }
   };
@@ -1165,19 +1168,22 @@ void scatter_hex8_kernel(ElementArgs args, FusedArgs f, int colour) {
   __shared__ float srow[EPW][8 * NF];
   const int grp = threadIdx.x / 8, a = threadIdx.x % 8;
   const int nex = f.nx - 1, ney = f.ny - 1, nez = f.nz - 1;
-  // element of this lane group: all elements in id order (atomics), or the colour's sub-grid
-  int64_t idx = (int64_t)blockIdx.x * EPW + grp;
+  // element of this lane group: all elements in id order (atomics), or the colour's sub-grid; 32-bit index arithmetic
+  // (fewer than 2^31 elements, checked at launch)
+  const uint32_t idx = blockIdx.x * (uint32_t)EPW + (uint32_t)grp;
   int ex, ey, ez;
   bool valid;
   if (ATOMIC) {
-    valid = idx < (int64_t)nex * ney * nez;
-    ez = (int)(idx % nez); ey = (int)((idx / nez) % ney); ex = (int)(idx / ((int64_t)nez * ney));
+    valid = idx < (uint32_t)nex * (uint32_t)ney * (uint32_t)nez;
+    const uint32_t q = idx / (uint32_t)nez;
+    ez = (int)(idx - q * (uint32_t)nez); ey = (int)(q % (uint32_t)ney); ex = (int)(q / (uint32_t)ney);
   } else {
     const int cx = colour & 1, cy = (colour >> 1) & 1, cz = (colour >> 2) & 1;
     const int mx = (nex - cx + 1) / 2, my = (ney - cy + 1) / 2, mz = (nez - cz + 1) / 2;
-    valid = mx > 0 && my > 0 && mz > 0 && idx < (int64_t)mx * my * mz;
-    const int iz = mz > 0 ? (int)(idx % mz) : 0, iy = my > 0 ? (int)((idx / mz) % my) : 0;
-    const int ix = (mz > 0 && my > 0) ? (int)(idx / ((int64_t)mz * my)) : 0;
+    valid = mx > 0 && my > 0 && mz > 0 && idx < (uint32_t)mx * (uint32_t)my * (uint32_t)mz;
+    const uint32_t q = mz > 0 ? idx / (uint32_t)mz : 0;
+    const int iz = mz > 0 ? (int)(idx - q * (uint32_t)mz) : 0, iy = my > 0 ? (int)(q % (uint32_t)my) : 0;
+    const int ix = my > 0 ? (int)(q / (uint32_t)my) : 0;
     ex = 2 * ix + cx; ey = 2 * iy + cy; ez = 2 * iz + cz;
   }
   if (!valid) { ex = ey = ez = 0; }
@@ -1239,6 +1245,7 @@ int launch_scatter_hex8(int nf, int scatter, const ElementArgs& a, const FusedAr
     }
   };
   if (scatter != 1 && scatter != 2) return -1;
+  if (nex * ney * nez >= (int64_t(1) << 31)) return -1;  // the kernels' 32-bit element index
   if (nf == 1 && phys == FECX_POISSON) {
     if (scatter == 1) go(scatter_hex8_kernel<1, FECX_POISSON, 1>); else go(scatter_hex8_kernel<1, FECX_POISSON, 0>);
     return 0;
