@@ -736,7 +736,16 @@ static void set_material(fecx_ctx* ctx, int block, int kind, const double* props
     const bool nh = kind == FECX_NEOHOOKEAN || kind == FECX_NH_SURROGATE;
     const bool j2 = kind == FECX_J2 || kind == FECX_J2_SURROGATE;
     if (nh) require(nprops >= 2, FECX_ERR_ARG, "neo-Hookean needs {lambda, mu}");
-    if (j2) require(nprops >= 4, FECX_ERR_ARG, "J2 needs {E, nu, sigma_y, H}");
+    if (j2) {
+      require(nprops >= 4, FECX_ERR_ARG, "J2 needs {E, nu, sigma_y, H}");
+      // the derived constants of the return mapping, in FP64 once here instead of FP32 divisions at every point
+      const double E = props[0], nu = props[1], Hh = props[3];
+      const double mu = E / (2 * (1 + nu)), kap = E / (3 * (1 - 2 * nu));
+      mp.p[4] = (float)mu;
+      mp.p[5] = (float)kap;
+      mp.p[6] = (float)(1.0 / (2 * mu + 2 * Hh / 3));
+      mp.p[7] = (float)(1.0 / (1 + Hh / (3 * mu)));
+    }
     for (int b = 0; b < nb; ++b)
       if (block < 0 || b == block) { ctx->block_mp[b] = mp; ctx->block_set[b] = 1; }
     ctx->mp = ctx->block_mp[0];

@@ -380,9 +380,11 @@ struct Mat<FECX_NEOHOOKEAN> {
 // (11,21,31,22,32,33), then equivalent plastic strain. Tangent C = kappa I(x)I + 2 mu theta (I_sym - I(x)I/3)
 // - 2 mu thetabar n(x)n, so with m = n g: K_ab = b1 g_a (x) g_b + b2 ((g_a . g_b) I + g_b (x) g_a) - b3 m_a (x) m_b,
 // b1 = kappa - 2 mu theta / 3, b2 = mu theta, b3 = 2 mu thetabar. C = {n (11,22,33,12,23,13), b1, b2, b3}.
+// p = {E, nu, sigma_y, H, mu, kappa, 1 / (2 mu + 2 H / 3), 1 / (1 + H / (3 mu))}: the last four formed on the host
+// (fecx_set_material), so that the point update divides once (by |s|) instead of ten times.
 __device__ __forceinline__ void j2_update(const float* p, const float* H, const float* so, float* sn, float* P, float* C) {
-  const float E = p[0], nu = p[1], sy = p[2], Hh = p[3];
-  const float mu = E / (2 * (1 + nu)), kap = E / (3 * (1 - 2 * nu));
+  const float sy = p[2], Hh = p[3], mu = p[4], kap = p[5], idg = p[6], ithb = p[7];
+  constexpr float third = 1.0f / 3.0f;
   const float ep[9] = {so[0], so[1], so[2], so[1], so[3], so[4], so[2], so[4], so[5]};
   float ee[9];
 #pragma unroll
@@ -393,7 +395,8 @@ __device__ __forceinline__ void j2_update(const float* p, const float* H, const 
   float s[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) s[i] = 2 * mu * ee[i];
-  s[0] -= 2 * mu * tr / 3; s[4] -= 2 * mu * tr / 3; s[8] -= 2 * mu * tr / 3;
+  const float sv = 2 * mu * (tr * third);
+  s[0] -= sv; s[4] -= sv; s[8] -= sv;
   float sn2 = 0;
 #pragma unroll
   for (int i = 0; i < 9; ++i) sn2 += s[i] * s[i];
@@ -405,7 +408,7 @@ __device__ __forceinline__ void j2_update(const float* p, const float* H, const 
   float nh[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) nh[i] = s[i] * inv;
-  const float dg = plastic ? f / (2 * mu + 2.0f * Hh / 3.0f) : 0.0f;
+  const float dg = plastic ? f * idg : 0.0f;
   if (P) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) P[i] = s[i] - 2 * mu * dg * nh[i];
@@ -417,10 +420,10 @@ __device__ __forceinline__ void j2_update(const float* p, const float* H, const 
     sn[6] = so[6] + r23 * dg;
   }
   if (C) {
-    const float th = plastic ? 1.0f - 2 * mu * dg / snorm : 1.0f;
-    const float thb = plastic ? 1.0f / (1.0f + Hh / (3 * mu)) - (1.0f - th) : 0.0f;
+    const float th = plastic ? 1.0f - 2 * mu * dg * inv : 1.0f;
+    const float thb = plastic ? ithb - (1.0f - th) : 0.0f;
     C[0] = nh[0]; C[1] = nh[4]; C[2] = nh[8]; C[3] = nh[1]; C[4] = nh[5]; C[5] = nh[2];
-    C[6] = kap - 2 * mu * th / 3; C[7] = mu * th; C[8] = 2 * mu * thb;
+    C[6] = kap - 2 * mu * th * third; C[7] = mu * th; C[8] = 2 * mu * thb;
   }
 }
 
