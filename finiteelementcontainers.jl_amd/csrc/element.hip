@@ -240,6 +240,15 @@ constexpr Tet10Addr make_tet10_addr() {
   return t;
 }
 __constant__ Tet10Addr c_tet10_addr = make_tet10_addr();
+constexpr int TET10_TR_JOB = 2;  // the one job of the tet10 lanes whose blocks are stored transposed
+constexpr bool tet10_transposes_one_job() {
+  const Tet10Addr t = make_tet10_addr();
+  for (int lane = 0; lane < 16; ++lane)
+    for (int p = 0; p < 4; ++p)
+      if (p != TET10_TR_JOB && ((t.stage[lane] >> (8 * p)) & 64u)) return false;
+  return true;
+}
+static_assert(tet10_transposes_one_job(), "tet10 staging: only job TET10_TR_JOB is stored transposed");
 
 // One lane's gather inputs: its local node's coordinates (hi/lo FP32 split) and field values. Loaded apart from the
 // element computation so that a caller can issue the loads early (the fused kernel fetches the next layer's while
@@ -725,7 +734,7 @@ __device__ __forceinline__ void element_eval(const ElementArgs& args, int64_t e,
       if constexpr (KIND == FECX_TET10) {
         const uint32_t t = (stage_addr >> (8 * p)) & 0xFFu;
         ok = (t & 128u) != 0;
-        tr = (t & 64u) != 0;
+        tr = p == TET10_TR_JOB && (t & 64u) != 0;  // no other job is stored transposed (checked with the table)
         pos = (int)(t & 63u);
       } else {
         ok = PJ::job(lane, p, b, slot, tr);
@@ -840,8 +849,13 @@ __global__ __launch_bounds__(256) void element_kernel(ElementArgs args) {
       // nontemporal: the blocks are read once, by the reduction (measured -2.5 % on the reduction that follows)
       if (((e0 * EW) & 3) == 0) {  // 16-byte aligned in HBM too (always, unless a pipelined piece starts unaligned)
         done = total & ~3;
-        for (int i = 4 * wl; i < done; i += 256)
-          __builtin_nontemporal_store(*reinterpret_cast<const f32x4*>(src + i), reinterpret_cast<f32x4*>(out + i));
+        constexpr int NIT = (EPWAVE * EW / 4 + 63) / 64;  // 16-byte pieces per lane for a full wavefront
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+          const int i = 4 * wl + 256 * it;
+          if (i < done)
+            __builtin_nontemporal_store(*reinterpret_cast<const f32x4*>(src + i), reinterpret_cast<f32x4*>(out + i));
+        }
       }
       for (int idx = done + wl; idx < total; idx += 64) out[idx] = src[idx];
     }
