@@ -12,8 +12,8 @@
 //   reduce_scalar_hex8_kernel   scalar hex8 light rows: lanes add their words into LDS slot accumulators;
 //   reduce_scatter_kernel       3-field light rows of up to 32 blocks, and the tet10 heavy rows (scalar and 3-field)
 //                               of up to 72 blocks: the same scatter, 32 or 64 lanes per row;
-//   reduce_merged_kernel        3-field tet10 meshes: both buckets of reduce_scatter_kernel in one launch, their
-//                               work-groups interleaved along the Morton curve;
+//   reduce_merged_kernel        3-field tet10 meshes whose rows have at most 65 blocks: both buckets of
+//                               reduce_scatter_kernel in one launch, their work-groups interleaved along the Morton curve;
 //   reduce_rows_kernel          slot-owner lanes for everything else: light scalar tet10/quad4 rows, 3-field light
 //                               rows of more than 32 blocks, heavy hex8/quad4 rows and tet10 rows of more than 72;
 //   reduce_vector_kernel / reduce_vector_group_kernel   calls without CSR values (one thread, or 16 lanes, per row).
@@ -314,13 +314,21 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scalar_hex8_kernel(ReduceA
 template <int NF, int NNPE, int LPR, int CHK, int SMAX>
 struct ScatterLDS {
   static constexpr int NPW = RED_THREADS / LPR, NB = NF * NF, VW = (NNPE + 3) / 4, BW = NNPE * NB;
-  int16_t s_off[NNPE][BW];
+  // block (a, b) of the staged rows: first word of its block in the element storage, bit 15 set when the storage holds
+  // K_ba (the words then transposed); 2 * NNPE^2 bytes instead of 2 * NNPE * BW for a table of every word
+  uint16_t s_blk[NNPE][NNPE];
   int32_t s_v[NPW][CHK];
   uint32_t s_w[NPW][CHK][VW];
   float s_r[NPW][CHK][NF];
   double s_acc[NPW][SMAX * NB];
-  int s_maxpass;
 };
+// A row's lanes lie in one wavefront and share LDS with no other row: between its staging and its sums, ordering
+// comes from the program order of the wavefront's LDS operations, which complete in order, kept by a wavefront-scope
+// fence, so the rows of a work-group never wait for each other.
+__device__ __forceinline__ void row_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 // The work-group's rows are the processing positions first + (thread / LPR), those below `end`.
 template <int NF, int NNPE, int LPR, int CHK, int SMAX>
 __device__ __forceinline__ void scatter_rows(const ReduceArgs& a, int64_t first, int64_t end, bool onepass,
@@ -336,13 +344,16 @@ __device__ __forceinline__ void scatter_rows(const ReduceArgs& a, int64_t first,
   // reduction 5.72 against 5.85 ms with 4), 4 on hex8 ones (neo-Hookean hex8 96^3: 0.983 against 1.001 ms with 2)
   constexpr int LB = JN <= 2 ? RED_LB1 : (NNPE == 10 ? 2 : RED_LB);
   static_assert(CHK <= LPR && CHK % LB == 0, "a row's lanes stage its incidences, LB per batch");
-  auto& s_off = L.s_off;
+  auto& s_blk = L.s_blk;
   auto& s_v = L.s_v;
   auto& s_w = L.s_w;
   auto& s_r = L.s_r;
   auto& s_acc = L.s_acc;
-  int& s_maxpass = L.s_maxpass;
-  for (int i = threadIdx.x; i < NNPE * BW; i += RED_THREADS) (&s_off[0][0])[i] = a.kel_off[i];
+  for (int i = threadIdx.x; i < NNPE * NNPE; i += RED_THREADS) {  // from the word table: words 0 and 1 of each block
+    const int16_t* o = a.kel_off + (i / NNPE) * BW + (i % NNPE) * NB;
+    const bool tr = NF > 1 && o[1] != o[0] + 1;
+    (&s_blk[0][0])[i] = (uint16_t)(o[0] | (tr ? 0x8000 : 0));
+  }
   const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
   const int64_t pos = first + grp;
   const bool valid = pos < end;
@@ -351,21 +362,17 @@ __device__ __forceinline__ void scatter_rows(const ReduceArgs& a, int64_t first,
   const int32_t c0 = ri.y, ninc = ri.w & 0xFFFF;
   const int deg = a.vals ? (ri.w >> 16) : 0;  // 0: no CSR blocks (a node with Dirichlet dofs)
   for (int k = lane; k < deg * NB; k += LPR) s_acc[grp][k] = 0.0;
-  int maxpass = 1;  // light rows (at most CHK incidences) take one pass
-  if (!onepass) {
-    if (threadIdx.x == 0) s_maxpass = 0;
-    __syncthreads();
-    if (lane == 0) atomicMax(&s_maxpass, (ninc + CHK - 1) / CHK);
-  }
-  __syncthreads();  // s_off
-  if (!onepass) maxpass = s_maxpass;
-  // this lane's words: block and entry of word jj * 64 + lane
-  int wb[JN], wc[JN];
+  // passes of this row (light rows, at most CHK incidences, take one); the two rows of a 32-lane wavefront may differ
+  const int maxpass = onepass ? 1 : (ninc + CHK - 1) / CHK;
+  __syncthreads();  // s_blk, the work-group's only barrier
+  // this lane's words: block and entry of word jj * LPR + lane, and the entry's place in the transposed block
+  int wb[JN], wc[JN], wt[JN];
 #pragma unroll
   for (int jj = 0; jj < JN; ++jj) {
     const int j = jj * LPR + lane;
     wb[jj] = j < BW ? j / NB : 0;
     wc[jj] = j % NB;
+    wt[jj] = (wc[jj] % NF) * NF + wc[jj] / NF;
   }
   double racc = 0.0;
   for (int pass = 0; pass < maxpass; ++pass) {
@@ -383,7 +390,7 @@ __device__ __forceinline__ void scatter_rows(const ReduceArgs& a, int64_t first,
         for (int i = 0; i < NF; ++i) s_r[grp][lane][i] = a.Rel[(int64_t)v * NF + i];
       }
     }
-    __syncthreads();
+    row_sync();  // s_v, s_w and s_r staged by the row's lanes
     if (a.R && lane < NF) {
       for (int t = 0; t < cnt; ++t) racc += (double)s_r[grp][t][lane];
     }
@@ -400,8 +407,9 @@ __device__ __forceinline__ void scatter_rows(const ReduceArgs& a, int64_t first,
           const int la = (int)(v % NNPE);
 #pragma unroll
           for (int jj = 0; jj < JN; ++jj) {
-            const int j = jj * LPR + lane;
-            w[tt][jj] = a.Kel[ebase + (j < BW ? (uint32_t)s_off[la][j] : 0u)];
+            const uint32_t e = s_blk[la][wb[jj]];
+            const uint32_t off = (e & 0x7FFFu) + (uint32_t)((e & 0x8000u) ? wt[jj] : wc[jj]);
+            w[tt][jj] = a.Kel[ebase + (jj * LPR + lane < BW ? off : 0u)];
           }
         }
 #pragma unroll
@@ -420,7 +428,7 @@ __device__ __forceinline__ void scatter_rows(const ReduceArgs& a, int64_t first,
         }
       }
     }
-    if (pass + 1 < maxpass) __syncthreads();  // the next pass restages s_v, s_w and s_r
+    row_sync();  // the next pass restages s_v, s_w and s_r
   }
   if (valid && a.R && lane < NF) a.R[n * NF + lane] = racc;
   // one wavefront wrote this row's accumulators, in order: no barrier before reading them back
@@ -444,6 +452,7 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scatter_kernel(ReduceArgs 
 }
 
 constexpr int HEAVY_SCATTER_SMAX = 72;  // structured tet10 vertex rows: 65 blocks
+constexpr int MERGED_HEAVY_SMAX = 65;   // the same rows in reduce_merged_kernel, whose LDS then fits 7 work-groups per CU
 
 // tet10 rows of 3 fields, light (edge) and heavy (vertex) rows in one launch: work-group w takes merged chunk w (after
 // the XCD remap), RED_MERGE_LIGHT edge rows on 32 lanes each or RED_MERGE_HEAVY vertex rows on 64 lanes each (the two
@@ -455,13 +464,14 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_merged_kernel(ReduceArgs a
   static_assert(RED_THREADS / 32 == RED_MERGE_LIGHT && RED_THREADS / 64 == RED_MERGE_HEAVY, "chunk sizes");
   __shared__ union {
     ScatterLDS<3, 10, 32, CH, 32> lo;
-    ScatterLDS<3, 10, 64, RED_HEAVY_CH, HEAVY_SCATTER_SMAX> hi;
+    ScatterLDS<3, 10, 64, RED_HEAVY_CH, MERGED_HEAVY_SMAX> hi;
   } L;
+  static_assert(sizeof(L) <= 163840 / 7, "7 work-groups per CU");
   const int64_t c = xcd_chunk(blockIdx.x, gridDim.x);
   if (c >= a.nmerged) return;  // the grid is rounded up to a multiple of 8
   const int2 ch = a.merged[c];
   if (ch.y == 0) scatter_rows<3, 10, 32, CH, 32>(a, ch.x, a.nlight, false, L.lo);
-  else scatter_rows<3, 10, 64, RED_HEAVY_CH, HEAVY_SCATTER_SMAX>(a, ch.x, a.p1, false, L.hi);
+  else scatter_rows<3, 10, 64, RED_HEAVY_CH, MERGED_HEAVY_SMAX>(a, ch.x, a.p1, false, L.hi);
 }
 
 // One launch over processing positions [a.p0, a.p1) whose rows have at most `maxdeg` blocks. Lanes per row: 8 for
@@ -528,7 +538,7 @@ static int launch_buckets(const ReduceArgs& a, hipStream_t s) {
     }
   } else if constexpr (NF == 3) {
     if (NNPE == 10 && a.merged && a.nmerged > 0 && a.p0 == 0 && a.nlight < a.p1 && a.maxdeg_light <= 32 &&
-        a.maxdeg <= HEAVY_SCATTER_SMAX) {
+        a.maxdeg <= MERGED_HEAVY_SMAX) {
       const int64_t grid = (a.nmerged + 7) / 8 * 8;  // XCD remap needs a multiple of 8
       reduce_merged_kernel<<<(unsigned)grid, RED_THREADS, 0, s>>>(a);
       return 0;
