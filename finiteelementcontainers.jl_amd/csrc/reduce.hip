@@ -319,7 +319,10 @@ struct ScatterLDS {
   uint16_t s_blk[NNPE][NNPE];
   int32_t s_v[NPW][CHK];
   uint32_t s_w[NPW][CHK][VW];
-  float s_r[NPW][CHK][NF];
+  // staged residual rows; 64-lane rows (one per wavefront, so a uniform incidence index) read them from the staging
+  // lanes' registers instead
+  static constexpr bool STAGE_R = LPR < 64;
+  float s_r[STAGE_R ? NPW : 1][STAGE_R ? CHK : 1][NF];
   double s_acc[NPW][SMAX * NB];
 };
 // A row's lanes lie in one wavefront and share LDS with no other row: between its staging and its sums, ordering
@@ -348,6 +351,7 @@ __device__ __forceinline__ void scatter_rows(const ReduceArgs& a, int64_t first,
   auto& s_v = L.s_v;
   auto& s_w = L.s_w;
   auto& s_r = L.s_r;
+  constexpr bool STAGE_R = ScatterLDS<NF, NNPE, LPR, CHK, SMAX>::STAGE_R;
   auto& s_acc = L.s_acc;
   for (int i = threadIdx.x; i < NNPE * NNPE; i += RED_THREADS) {  // from the word table: words 0 and 1 of each block
     const int16_t* o = a.kel_off + (i / NNPE) * BW + (i % NNPE) * NB;
@@ -378,6 +382,7 @@ __device__ __forceinline__ void scatter_rows(const ReduceArgs& a, int64_t first,
   for (int pass = 0; pass < maxpass; ++pass) {
     const int base = pass * CHK;
     const int cnt = min(CHK, ninc - base);  // may be <= 0 for this node
+    float rr[NF] = {};  // 64-lane rows: this lane's incidence's residual row
     if (lane < cnt) {
       const int64_t c = (int64_t)c0 + base + lane;
       const int32_t v = a.vent[c];
@@ -387,12 +392,28 @@ __device__ __forceinline__ void scatter_rows(const ReduceArgs& a, int64_t first,
       for (int q = 0; q < VW; ++q) s_w[grp][lane][q] = sl[q];
       if (a.R) {
 #pragma unroll
-        for (int i = 0; i < NF; ++i) s_r[grp][lane][i] = a.Rel[(int64_t)v * NF + i];
+        for (int i = 0; i < NF; ++i) {
+          const float r = a.Rel[(int64_t)v * NF + i];
+          if constexpr (STAGE_R) s_r[grp][lane][i] = r;
+          else rr[i] = r;
+        }
       }
     }
     row_sync();  // s_v, s_w and s_r staged by the row's lanes
-    if (a.R && lane < NF) {
-      for (int t = 0; t < cnt; ++t) racc += (double)s_r[grp][t][lane];
+    if constexpr (STAGE_R) {
+      if (a.R && lane < NF) {
+        for (int t = 0; t < cnt; ++t) racc += (double)s_r[grp][t][lane];
+      }
+    } else if (a.R) {
+      for (int t = 0; t < cnt; ++t) {  // lane i < NF adds entry i of incidence t's row, held by lane t
+        float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rr[0]), t));
+#pragma unroll
+        for (int i = 1; i < NF; ++i) {
+          const float xi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rr[i]), t));
+          if (lane == i) x = xi;
+        }
+        racc += (double)x;
+      }
     }
     if (deg > 0) {
 #pragma unroll
@@ -452,7 +473,7 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_scatter_kernel(ReduceArgs 
 }
 
 constexpr int HEAVY_SCATTER_SMAX = 72;  // structured tet10 vertex rows: 65 blocks
-constexpr int MERGED_HEAVY_SMAX = 65;   // the same rows in reduce_merged_kernel, whose LDS then fits 7 work-groups per CU
+constexpr int MERGED_HEAVY_SMAX = 65;   // the same rows in reduce_merged_kernel, whose LDS then fits 8 work-groups per CU
 
 // tet10 rows of 3 fields, light (edge) and heavy (vertex) rows in one launch: work-group w takes merged chunk w (after
 // the XCD remap), RED_MERGE_LIGHT edge rows on 32 lanes each or RED_MERGE_HEAVY vertex rows on 64 lanes each (the two
@@ -460,13 +481,14 @@ constexpr int MERGED_HEAVY_SMAX = 65;   // the same rows in reduce_merged_kernel
 // own launch reads the blocks its rows share with the other bucket's rows (24 of an element's 55) a second time,
 // long after the first; here the vertex and edge rows of the same elements run at about the same time on the same
 // XCD. Every row's sums are those of the bucket kernels, so the bits are too.
-__global__ __launch_bounds__(RED_THREADS) void reduce_merged_kernel(ReduceArgs a) {
+__global__ __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void reduce_merged_kernel(
+    ReduceArgs a) {
   static_assert(RED_THREADS / 32 == RED_MERGE_LIGHT && RED_THREADS / 64 == RED_MERGE_HEAVY, "chunk sizes");
   __shared__ union {
     ScatterLDS<3, 10, 32, CH, 32> lo;
     ScatterLDS<3, 10, 64, RED_HEAVY_CH, MERGED_HEAVY_SMAX> hi;
   } L;
-  static_assert(sizeof(L) <= 163840 / 7, "7 work-groups per CU");
+  static_assert(sizeof(L) <= 163840 / 8, "8 work-groups per CU");
   const int64_t c = xcd_chunk(blockIdx.x, gridDim.x);
   if (c >= a.nmerged) return;  // the grid is rounded up to a multiple of 8
   const int2 ch = a.merged[c];
